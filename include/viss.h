@@ -1,0 +1,145 @@
+/*
+ * viss.h -- C ABI of the MI355X (gfx950) lunar-orbit interferometer hot path:
+ * HEALPix pix2ang, the direct-sum 3D visibility generator (forward), the
+ * density-compensation weights and the back-projection / adjoint.
+ *
+ * Conventions (all entry points):
+ *   - every array argument is a DEVICE pointer on the current HIP device,
+ *     owned by the caller; `stream` is a hipStream_t (NULL = default stream);
+ *   - calls are asynchronous on `stream` and never synchronise;
+ *   - the return value is a status (VISS_OK = 0, negative on error); the
+ *     message of the last failure on the calling thread is viss_last_error();
+ *   - "SoA 3xN" means three consecutive blocks of N doubles (x[0..N), y, z);
+ *   - complex arrays are interleaved (re, im) doubles (std::complex<double>);
+ *   - output kernels ACCUMULATE (+=) into the caller's zeroed buffers, as the
+ *     reference's thrust::device_vector outputs do.
+ *
+ * The reference exposes no library: its boundary is the pair of executables
+ * pix2ang.cu / vissgen3d_*.cu and the raw-pointer kernels they launch.  Each
+ * entry point below names the reference kernel(s) and launch site it replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef VISS_H
+#define VISS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VISS_VERSION 100
+
+/* status codes */
+#define VISS_OK 0
+#define VISS_ERR_INVALID (-1) /* bad argument (null pointer, size, flag) */
+#define VISS_ERR_HIP (-2)     /* HIP runtime / launch error */
+#define VISS_ERR_SIZE (-3)    /* size overflow or above a stated limit */
+#define VISS_ERR_NOMEM (-4)   /* device allocation failed */
+
+/* flags */
+#define VISS_MASK 1               /* lunar occultation mask m_kp */
+#define VISS_PHASE_TRACK 2        /* forward: output phase u l + v m + w (n-1); backproject: input is phase-tracked */
+#define VISS_DCF_BLOCKAGE_SHIFT 4 /* weights: vissgen3d_blockage.cu:138 indexing of dcf */
+#define VISS_PREC_FP64 8          /* all-FP64 kernels (default: FP64 phase, FP32 sincos/accumulate, FP64 combine) */
+#define VISS_HERMITIAN 16         /* rows k+K are (-b_k, conj): forward writes them, backproject folds them in */
+
+/* pix2ang schemes */
+#define VISS_SCHEME_RING 0
+#define VISS_SCHEME_NEST 1
+
+/* Occultation geometry.  Pixel direction s is visible from satellite r iff
+ *   s . r >= t(r),  t(r) = cos(Phi) |r|,  Phi = pi - asin(R_moon / orbit_radius)   if orbit_radius > 0
+ *                   t(r) = -sqrt(|r|^2 - R_moon^2)                                    otherwise.
+ * orbit_radius > 0 is the reference's fixed cut (vissgen3d_noblockage.cu:240-243,
+ * R = 1737.1 km, R + h = 2037.1 km); <= 0 uses each satellite's own |r|. */
+typedef struct viss_mask_geometry {
+  double R_moon;
+  double orbit_radius;
+} viss_mask_geometry;
+
+int viss_version(void);
+const char* viss_status_string(int status);
+const char* viss_last_error(void);
+
+/* HEALPix pix2ang for all 12 nside^2 pixels.  Replaces mk_pix2xy +
+ * f_pix2and_nest (pix2ang.cu:61-160), launched by vissGen (pix2ang.cu:
+ * 190-264); adds the RING scheme.  theta, phi: [P] FP64 (colatitude in
+ * [0,pi], longitude in [0,2pi)); ring, iphi: [P] int32 1-based ring number
+ * from the north pole and 1-based index within the ring (nullable).
+ * 1 <= nside <= 8192. */
+int viss_pix2ang(int64_t nside, int scheme, double* theta, double* phi, int32_t* ring, int32_t* iphi,
+                 void* stream);
+
+/* Direction cosines and scaled brightness, healpix_moonback_pre
+ * (vissgen3d_noblockage.cu:52-73, launch :404-415):
+ *   l = sin(theta)cos(phi), m = -sin(theta)sin(phi), n = cos(theta), I = B * scale.
+ * lmn: SoA 3xP FP64 output; B / I nullable (both or neither). */
+int viss_pixel_dirs(const double* theta, const double* phi, const double* B, int64_t P, double scale, double* lmn,
+                    double* I, void* stream);
+
+/* Forward direct sum, healpix_moonback_viss (vissgen3d_noblockage.cu:76-102,
+ * launch :418-440):
+ *   V_k += sum_p [mask] I_p exp(-2 pi i (u_k l_p + v_k m_p + w_k n_p [- w_k with VISS_PHASE_TRACK]))
+ * lmn: SoA 3xP; I: [P]; uvw: SoA 3xK [wavelengths]; xyz_a/xyz_b: SoA 3xK [m]
+ * (only read with VISS_MASK); V: complex [K] ([2K] with VISS_HERMITIAN, rows
+ * k+K receiving conj(V_k), vissgen3d_noblockage.cu:100).  geom is read with
+ * VISS_MASK.  Builds and frees a temporary sky (see viss_sky_create). */
+int viss_forward(const double* lmn, const double* I, int64_t P, const double* uvw, const double* xyz_a,
+                 const double* xyz_b, int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream);
+
+/* Back-projection / adjoint, computeC (vissgen3d_noblockage.cu:167-182,
+ * masked form vissgen3d_blockage.cu:167-192, launch :511-527):
+ *   x_p += herm * Re sum_k [mask] wt_k y'_k exp(+2 pi i (u_k l_p + v_k m_p + w_k n_p))
+ * y'_k = y_k, or y_k exp(-2 pi i w_k) with VISS_PHASE_TRACK (the phase restore
+ * of viss_gamma_trans, vissgen3d_noblockage.cu:149-151); wt: [K] FP64 or
+ * NULL (unit weights: the adjoint A^H of viss_forward); herm = 2 with
+ * VISS_HERMITIAN (the K rows are the first half of a Hermitian set), else 1.
+ * y: complex [K]; x: [P] FP64. */
+int viss_backproject(const double* lmn, int64_t P, const double* uvw, const double* xyz_a, const double* xyz_b,
+                     const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
+                     double* x, void* stream);
+
+/* Density-compensation weights, vissgen3d_noblockage.cu:105-164 (ceilAndScale,
+ * thrust::sort, countOccurrences, calculateR, calculateDcf, viss_gamma_trans;
+ * launches :448-507):
+ *   gs_k = ceil((bll_k - 1/4) / (1/2))  (FP32, as the reference)   -> gs [K] int32
+ *   mb_j = #{k : gs_k = j}, j = 1..nr                               -> mb [nr] uint32 (mb[j-1])
+ *   dcf_0 = 1/(4 pi), dcf_j = (2 pi/3)(R^3 - (R-1/2)^3)/mb_j, R = j/2 + 1/4  -> dcf [nr+1] FP64
+ *   dg_k = 1.5 sqrt|1 - 4 sin^2 gamma| / |cos gamma|, gamma = asin(w/bll)   -> dg [K] FP64 (nullable)
+ *   wt_k = fmin(dcf[gs_k] dg_k, 1/8)  (NaN -> 1/8)                  -> wt [K] FP64
+ * Rows with gs outside [0, nr] (out-of-bounds reads in the reference) get
+ * wt = 0 and are counted into *n_out_of_range (device int64, accumulated,
+ * nullable).  VISS_DCF_BLOCKAGE_SHIFT: dcf_j uses shell j+1
+ * (vissgen3d_blockage.cu:138) and dcf_nr = 0.  mb must be zeroed by the caller
+ * (it accumulates).  1 <= nr <= 2^26. */
+int viss_weights(const float* bll, const double* w, int64_t K, int32_t nr, int flags, int32_t* gs, uint32_t* mb,
+                 double* dcf, double* dg, double* wt, int64_t* n_out_of_range, void* stream);
+
+/* ---- reusable sky: the pixel set sorted into spatially compact 64-pixel
+ * tiles with bounding cones, packed for the kernels.  Building it costs a
+ * sort; viss_forward / viss_backproject build a temporary one per call.  A
+ * sky is bound to the device current at creation; calls that share one sky
+ * must be ordered (same stream or synchronised). */
+typedef struct viss_sky viss_sky;
+
+int viss_sky_create(const double* lmn, int64_t P, viss_sky** sky, void* stream);
+int viss_sky_destroy(viss_sky* sky);
+int64_t viss_sky_npix(const viss_sky* sky);
+
+int viss_sky_forward(viss_sky* sky, const double* I, const double* uvw, const double* xyz_a, const double* xyz_b,
+                     int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream);
+int viss_sky_backproject(viss_sky* sky, const double* uvw, const double* xyz_a, const double* xyz_b,
+                         const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
+                         double* x, void* stream);
+
+/* Number of (k, p) pairs with mask = 1 (accumulated into *count, device
+ * int64): the evaluated-pair count used for throughput accounting. */
+int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,
+                           const viss_mask_geometry* geom, int64_t* count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VISS_H */

@@ -259,7 +259,11 @@ void oracle_forward_ref32(const float* l, const float* m, const float* n, const 
 int32_t oracle_shell_index(float bll) {
   volatile float d = bll - 0.25f;
   volatile float q = d / 0.5f;
-  return (int32_t)ceilf(q);
+  const float c = ceilf(q);
+  /* NaN and values outside int32 (undefined casts in the reference) map to
+   * INT32_MIN: out of range, weight 0. */
+  if (!(c > -2147483648.0f && c < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)c;
 }
 
 /* Density-compensation weights, vissgen3d_noblockage.cu:105-164:

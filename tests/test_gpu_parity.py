@@ -1,0 +1,211 @@
+"""GPU parity: the HIP library through its C ABI against the FP64 oracle on the
+same seeded inputs.
+
+Tolerances (stated here, per quantity):
+  * pix2ang ring / in-ring indices and phi: bit-exact; theta within 1 ulp;
+  * pixel directions: 4e-16 absolute;
+  * weights: gs, mb, dcf bit-exact; dg, wt within 1e-14 relative or 1e-13
+    absolute (dg = 1.5 sqrt|1 - 4 sin^2 g| / |cos g| loses digits by
+    cancellation where sin g ~ 1/2, so device and host libm ulps are amplified);
+  * all-FP64 forward / back-projection: relative L2 <= 1e-11 (FP64 phase
+    rounding alone is ~1e-12 relative at |b.s| ~ 10^4 cycles);
+  * mixed-precision forward / back-projection: relative L2 <= 1e-5 (the
+    precision target) -- the measured value is printed;
+  * visible-pair counts: exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.cases import make_case, rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+R_MOON = 1737.4e3
+FP64_TOL = 1e-11
+MIXED_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from viss_optimize_3d_amd import ops as _ops
+
+    return _ops
+
+
+def cuda(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to("cuda", dtype) if dtype is not None else t.to("cuda")
+
+
+@pytest.mark.parametrize("nest", [False, True])
+@pytest.mark.parametrize("nside", [1, 2, 4, 16, 64, 256, 1024])
+def test_pix2ang_parity(ops, nside, nest):
+    th, ph, ring, iphi = ops.pix2ang(nside, nest=nest)
+    oth, oph, oring, oiphi = oracle.pix2ang(nside, nest)
+    np.testing.assert_array_equal(ring.cpu().numpy(), oring)
+    np.testing.assert_array_equal(iphi.cpu().numpy(), oiphi)
+    np.testing.assert_array_equal(ph.cpu().numpy(), oph)
+    d = np.abs(th.cpu().numpy() - oth)
+    assert np.all(d <= np.spacing(oth)), f"theta off by {np.max(d / np.spacing(oth))} ulp"
+
+
+def test_pix2ang_rejects_bad_nside(ops):
+    from viss_optimize_3d_amd._lib import VissError
+
+    with pytest.raises(VissError):
+        ops.pix2ang(0)
+    with pytest.raises(VissError):
+        ops.pix2ang(12, nest=True)  # NEST needs a power of two
+
+
+def test_pixel_dirs_parity(ops):
+    th, ph, _, _ = oracle.pix2ang(64, True)
+    B = np.random.default_rng(0).lognormal(size=th.size)
+    lmn, I = ops.pixel_dirs(cuda(th), cuda(ph), cuda(B))
+    l, m, n, oI = oracle.pixel_dirs(th, ph, B, 4 * np.pi / th.size)
+    np.testing.assert_allclose(lmn.cpu().numpy(), np.stack([l, m, n]), rtol=0, atol=4e-16)
+    np.testing.assert_allclose(I.cpu().numpy(), oI, rtol=1e-15)
+
+
+CASES = {
+    # plain method: nside 16, 4 satellites (6 baselines) x 500 snapshots over one orbit, 1 MHz, no mask
+    "plain-nside16": dict(nside=16, n_snap=500, n_sat=4, freq=1e6, sigma=10e3, rotating=False, snaps_per_orbit=500,
+                          seed=1),
+    # blockage: nside 64, 8 satellites, rotating orbit normal (a 2,800-row slice of the 280,000)
+    "blockage-nside64": dict(nside=64, n_snap=100, n_sat=8, freq=1e6, sigma=10e3, rotating=True,
+                             snaps_per_orbit=100, seed=2),
+    # precision: nside 256, 10 MHz, sigma 50 km (|b| to thousands of wavelengths), a 225-row slice
+    "precision-nside256": dict(nside=256, n_snap=5, n_sat=10, freq=10e6, sigma=50e3, rotating=True,
+                               snaps_per_orbit=100, seed=3),
+}
+MASKED = {"plain-nside16": False, "blockage-nside64": True, "precision-nside256": True}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    c = make_case(**CASES[request.param])
+    c["name"] = request.param
+    c["mask"] = MASKED[request.param]
+    return c
+
+
+@pytest.fixture(scope="module")
+def oracle_fwd(case):
+    flags = oracle.MASK if case["mask"] else 0
+    return oracle.forward(*case["lmn"], case["I"], case["uvw"], case["xa"], case["xb"], flags=flags, R_moon=R_MOON)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_forward_parity(ops, case, oracle_fwd, fp64):
+    sky = ops.Sky(cuda(case["lmn"]))
+    V = sky.forward(cuda(case["I"]), cuda(case["uvw"]), cuda(case["xa"]), cuda(case["xb"]), mask=case["mask"],
+                    R_moon=R_MOON, fp64=fp64)
+    err = rel_l2(V.cpu().numpy(), oracle_fwd)
+    print(f"\n{case['name']} forward fp64={fp64}: rel L2 {err:.3e}")
+    assert err <= (FP64_TOL if fp64 else MIXED_TOL)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_backproject_parity(ops, case, fp64):
+    rng = np.random.default_rng(9)
+    K = case["uvw"].shape[1]
+    y = rng.normal(size=K) + 1j * rng.normal(size=K)
+    wt = rng.uniform(0.0, 0.125, K)
+    flags = oracle.MASK if case["mask"] else 0
+    ref = oracle.backproject(*case["lmn"], case["uvw"], y, wt, case["xa"], case["xb"], flags=flags, R_moon=R_MOON)
+    sky = ops.Sky(cuda(case["lmn"]))
+    x = sky.backproject(cuda(case["uvw"]), cuda(y), cuda(wt), cuda(case["xa"]), cuda(case["xb"]), mask=case["mask"],
+                        R_moon=R_MOON, fp64=fp64)
+    err = rel_l2(x.cpu().numpy(), ref)
+    print(f"\n{case['name']} backproject fp64={fp64}: rel L2 {err:.3e}")
+    assert err <= (FP64_TOL if fp64 else MIXED_TOL)
+
+
+def test_forward_flags(ops):
+    """Phase tracking and Hermitian rows (vissgen3d_noblockage.cu:95,100) and the
+    fixed reference cut (orbit_radius > 0)."""
+    c = make_case(nside=16, n_snap=40, n_sat=5, freq=1e6, seed=4)
+    args = (*c["lmn"], c["I"], c["uvw"], c["xa"], c["xb"])
+    orbit = 1737.1e3 + 300e3
+    ref = oracle.forward(*args, flags=oracle.MASK | oracle.PHASE_TRACK | oracle.HERMITIAN, R_moon=1737.1e3,
+                         orbit_radius=orbit)
+    sky = ops.Sky(cuda(c["lmn"]))
+    for fp64 in (True, False):
+        V = sky.forward(cuda(c["I"]), cuda(c["uvw"]), cuda(c["xa"]), cuda(c["xb"]), mask=True, R_moon=1737.1e3,
+                        orbit_radius=orbit, phase_track=True, hermitian=True, fp64=fp64)
+        assert rel_l2(V.cpu().numpy(), ref) <= (FP64_TOL if fp64 else MIXED_TOL)
+    # accumulate semantics: a second call adds
+    V2 = sky.forward(cuda(c["I"]), cuda(c["uvw"]), cuda(c["xa"]), cuda(c["xb"]), mask=True, R_moon=1737.1e3,
+                     orbit_radius=orbit, phase_track=True, hermitian=True, fp64=True, out=V.clone())
+    assert rel_l2(V2.cpu().numpy(), 2 * ref) <= 1e-5
+
+
+def test_backproject_flags(ops):
+    c = make_case(nside=16, n_snap=40, n_sat=5, freq=1e6, seed=5)
+    K = c["uvw"].shape[1]
+    rng = np.random.default_rng(2)
+    y = rng.normal(size=K) + 1j * rng.normal(size=K)
+    f = oracle.MASK | oracle.PHASE_TRACK | oracle.HERMITIAN
+    ref = oracle.backproject(*c["lmn"], c["uvw"], y, None, c["xa"], c["xb"], flags=f, R_moon=R_MOON)
+    sky = ops.Sky(cuda(c["lmn"]))
+    for fp64 in (True, False):
+        x = sky.backproject(cuda(c["uvw"]), cuda(y), None, cuda(c["xa"]), cuda(c["xb"]), mask=True, R_moon=R_MOON,
+                            phase_track=True, hermitian=True, fp64=fp64)
+        assert rel_l2(x.cpu().numpy(), ref) <= (FP64_TOL if fp64 else MIXED_TOL)
+
+
+def test_adjoint_dot_product_gpu(ops):
+    c = make_case(nside=32, n_snap=50, n_sat=6, freq=10e6, sigma=30e3, seed=6)
+    K = c["uvw"].shape[1]
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=c["I"].size)
+    y = rng.normal(size=K) + 1j * rng.normal(size=K)
+    sky = ops.Sky(cuda(c["lmn"]))
+    for fp64, tol in ((True, 1e-11), (False, 1e-6)):
+        Ax = sky.forward(cuda(x), cuda(c["uvw"]), cuda(c["xa"]), cuda(c["xb"]), mask=True, R_moon=R_MOON, fp64=fp64)
+        Ahy = sky.backproject(cuda(c["uvw"]), cuda(y), None, cuda(c["xa"]), cuda(c["xb"]), mask=True, R_moon=R_MOON,
+                              fp64=fp64)
+        lhs = np.real(np.vdot(Ax.cpu().numpy(), y))
+        rhs = np.dot(x, Ahy.cpu().numpy())
+        scale = np.linalg.norm(Ax.cpu().numpy()) * np.linalg.norm(y)
+        assert abs(lhs - rhs) <= tol * scale
+
+
+def test_count_visible_exact(ops):
+    c = make_case(nside=64, n_snap=30, n_sat=8, freq=1e6, seed=7)
+    sky = ops.Sky(cuda(c["lmn"]))
+    cnt = int(sky.count_visible(cuda(c["xa"]), cuda(c["xb"]), R_moon=R_MOON).item())
+    assert cnt == oracle.count_visible(*c["lmn"], c["xa"], c["xb"], R_MOON)
+
+
+def test_weights_parity(ops):
+    rng = np.random.default_rng(8)
+    nr = 667
+    K = 200_000
+    bll = rng.uniform(0.0, 360.0, K).astype(np.float32)
+    bll[:5] = [0.25, 0.75, 1.25, 333.75, 2e6]
+    w = rng.uniform(-1, 1, K) * bll * 0.95
+    w[7] = 5 * bll[7]  # NaN dg -> weight 1/8
+    for shift in (False, True):
+        out = ops.weights(cuda(bll), cuda(w), nr, blockage_shift=shift)
+        gs, mb, dcf, dg, wt, bad = oracle.weights(bll, w, nr, oracle.DCF_BLOCKAGE_SHIFT if shift else 0)
+        np.testing.assert_array_equal(out["gs"].cpu().numpy(), gs)
+        np.testing.assert_array_equal(out["mb"].cpu().numpy().astype(np.uint32), mb)
+        np.testing.assert_array_equal(out["dcf"].cpu().numpy(), dcf)
+        np.testing.assert_allclose(out["dg"].cpu().numpy(), dg, rtol=1e-14, atol=1e-13, equal_nan=True)
+        np.testing.assert_allclose(out["wt"].cpu().numpy(), wt, rtol=1e-14, atol=1e-13)
+        assert int(out["n_out_of_range"].item()) == bad
+
+
+def test_empty_inputs(ops):
+    c = make_case(nside=4, n_snap=2, n_sat=3, seed=1)
+    sky = ops.Sky(cuda(c["lmn"]))
+    z = torch.zeros(3, 0, dtype=torch.float64, device="cuda")
+    V = sky.forward(cuda(c["I"]), z, z, z, mask=True)
+    assert V.numel() == 0
+    x = sky.backproject(z, torch.zeros(0, dtype=torch.complex128, device="cuda"), None, z, z, mask=True)
+    assert float(x.abs().sum()) == 0.0

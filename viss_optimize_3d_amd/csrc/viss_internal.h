@@ -1,0 +1,119 @@
+// Internal declarations shared by the HIP translation units of libviss_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "viss.h"
+
+namespace viss {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// Thread-local error message (viss_last_error) and helpers returning a status.
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define VISS_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t viss_e_ = (call);                                                \
+    if (viss_e_ != hipSuccess) return ::viss::hip_fail(viss_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define VISS_REQUIRE(cond, ...)          \
+  do {                                   \
+    if (!(cond)) {                       \
+      ::viss::set_error(__VA_ARGS__);    \
+      return VISS_ERR_INVALID;           \
+    }                                    \
+  } while (0)
+
+// Launch-error check after a kernel launch.
+#define VISS_LAUNCHED(name) VISS_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------- packing
+// One pixel of the sorted sky: direction (FP64), FP32 brightness slot written
+// per forward call, and the caller's index (-1 for padding).
+struct PixPack {
+  double l, m, n;
+  float I;
+  int32_t idx;
+};
+static_assert(sizeof(PixPack) == 32, "PixPack is 32 B");
+
+// Per-visibility data read wave-uniformly by the back-projection.
+struct VisHot {  // mixed precision: phase in FP64, weighted visibility in FP32
+  double u, v, w;
+  float yr, yi;
+};
+static_assert(sizeof(VisHot) == 32, "VisHot is 32 B");
+
+struct VisHot64 {  // all-FP64 path
+  double u, v, w, yr, yi, pad0, pad1, pad2;
+};
+static_assert(sizeof(VisHot64) == 64, "VisHot64 is 64 B");
+
+struct VisMask {  // exact occultation test: s.r_a >= ta && s.r_b >= tb (FP64)
+  double xa, ya, za, ta, xb, yb, zb, tb;
+};
+static_assert(sizeof(VisMask) == 64, "VisMask is 64 B");
+
+struct VisCls {  // tile classification: unit vectors and unit cuts (FP32)
+  float ax, ay, az, ca, bx, by, bz, cb;
+};
+static_assert(sizeof(VisCls) == 32, "VisCls is 32 B");
+
+// Tile classification margin (FP32 rounding of unit vectors, cuts and cone
+// data is below 1e-6; the margin keeps every "all visible" / "none visible"
+// verdict consistent with the exact FP64 per-pixel test).
+constexpr float kClsEps = 1e-5f;
+
+constexpr int kTile = 64;       // forward tile (pixels)
+constexpr int kBpPixPerLane = 4; // back-projection pixels per lane
+constexpr int kBpTile = 64 * kBpPixPerLane;
+constexpr int kPadTo = kBpTile;  // sky padding granularity
+
+// Occultation threshold t(r): s visible from r iff s.r >= t
+// (vissgen3d_noblockage.cu:88-94,240-243 in dot-product form).
+__host__ __device__ inline double mask_threshold(double x, double y, double z, double R_moon, double orbit_radius) {
+  const double r2 = x * x + y * y + z * z;
+  if (orbit_radius > 0.0) {
+    const double q = R_moon / orbit_radius;
+    return -sqrt(1.0 - q * q) * sqrt(r2);
+  }
+  const double d = r2 - R_moon * R_moon;
+  return d > 0.0 ? -sqrt(d) : 0.0;
+}
+
+// Device buffers growable in stream order.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t need, hipStream_t s);
+  void release();
+};
+
+}  // namespace viss
+
+struct viss_sky {
+  int device = -1;
+  int64_t P = 0;       // pixels
+  int64_t P_pad = 0;   // padded to kPadTo
+  int64_t ntile = 0;   // P_pad / kTile
+  int64_t nbtile = 0;  // P_pad / kBpTile
+  viss::PixPack* pix = nullptr;  // [P_pad] sorted
+  double* I64 = nullptr;         // [P_pad] FP64 brightness (all-FP64 forward)
+  float4* cone = nullptr;        // [ntile] bounding cones of kTile-pixel tiles
+  float4* bcone = nullptr;       // [nbtile] bounding cones of kBpTile-pixel tiles
+  int32_t* tile_valid = nullptr; // [ntile] number of real pixels per tile
+  viss::DevBuf ws;               // per-call visibility packs
+};
+
+namespace viss {
+int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s);
+void sky_free(viss_sky* sky);
+int sky_set_brightness(viss_sky* sky, const double* I, bool want64, hipStream_t s);
+int check_device(const viss_sky* sky);
+}  // namespace viss

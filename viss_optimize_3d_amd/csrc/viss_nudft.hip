@@ -1,0 +1,512 @@
+// Direct-sum 3D visibility generator (forward) and back-projection / adjoint.
+//
+// Reference: healpix_moonback_viss (vissgen3d_noblockage.cu:76-102) runs one
+// thread per visibility over every pixel with FP32 phase, acosf mask tests and
+// a read-modify-write of Viss[i] in HBM per pixel; computeC
+// (vissgen3d_noblockage.cu:167-182, masked vissgen3d_blockage.cu:167-192) runs
+// one thread per pixel over every visibility with a dependent dcf[gs[i]]
+// gather and a read-modify-write of C[idx] per pair.
+//
+// Here (gfx950, wave64):
+//  * forward: each lane owns one visibility (u, v, w in FP64 VGPRs, FP32/FP64
+//    accumulators in registers); every lane of a wavefront walks the same
+//    pixel, so pixel data arrive wave-uniformly through scalar loads
+//    (s_load) from the sorted sky and feed VALU ops as SGPR operands.  No
+//    cross-lane traffic inside the pair loop.
+//  * back-projection: each lane owns kBpPixPerLane pixels of one
+//    kBpTile-pixel tile; visibilities are walked wave-uniformly (scalar
+//    loads of 32-B records prepared by a prologue that folds the
+//    density-compensation weight and the phase restore into y).
+//  * occultation: per (lanes' visibilities, pixel tile) the tile's bounding
+//    cone gives "none visible" (the wavefront skips the tile), "all
+//    visible" (no per-pair test) or "partial" (exact FP64 test s.r >= t per
+//    pair).  The classification is conservative, so results equal the
+//    per-pair FP64 mask.
+//  * precision (mixed): phase b.s + c0 formed with FP64 FMAs, reduced with
+//    v_fract_f64 to [0,1) revolutions, converted to FP32 and fed to the
+//    hardware v_sin_f32 / v_cos_f32 (which take revolutions); products
+//    accumulate in FP32 over one tile (<= 64 terms) and the tile sums
+//    combine in FP64 (blocked summation).  Partial results of different
+//    pixel / visibility chunks meet through FP64 atomics in the caller's
+//    accumulate-into buffer.
+//  * VISS_PREC_FP64: same structure, FP64 sincospi and FP64 accumulation.
+#include "viss_internal.h"
+
+namespace viss {
+
+// ------------------------------------------------------------ pair math
+__device__ __forceinline__ float rev_fp32(double ph) {
+  return (float)__builtin_amdgcn_fract(ph);  // [0,1) revolutions, exact in FP64
+}
+
+__device__ __forceinline__ void sincos_rev(float r, float& s, float& c) {
+  s = __builtin_amdgcn_sinf(r);  // sin(2 pi r)
+  c = __builtin_amdgcn_cosf(r);
+}
+
+__device__ __forceinline__ bool both_visible(double l, double m, double n, const VisMask& q) {
+  const double da = fma(l, q.xa, fma(m, q.ya, n * q.za));
+  const double db = fma(l, q.xb, fma(m, q.yb, n * q.zb));
+  return (da >= q.ta) & (db >= q.tb);
+}
+
+// Conservative cone test of one visibility's two satellites against a tile.
+__device__ __forceinline__ void classify(const float4 cn, const VisCls& c, bool& any_v, bool& all_v) {
+  const float da = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+  const float db = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, cn.z * c.bz));
+  any_v = (da + cn.w >= c.ca - kClsEps) & (db + cn.w >= c.cb - kClsEps);
+  all_v = (da - cn.w >= c.ca + kClsEps) & (db - cn.w >= c.cb + kClsEps);
+}
+
+__device__ __forceinline__ void make_mask_records(double xa, double ya, double za, double xb, double yb, double zb,
+                                                  double R_moon, double orbit_r, VisMask& m, VisCls& c) {
+  m.xa = xa; m.ya = ya; m.za = za;
+  m.xb = xb; m.yb = yb; m.zb = zb;
+  m.ta = mask_threshold(xa, ya, za, R_moon, orbit_r);
+  m.tb = mask_threshold(xb, yb, zb, R_moon, orbit_r);
+  const double ra = sqrt(xa * xa + ya * ya + za * za), rb = sqrt(xb * xb + yb * yb + zb * zb);
+  const double ia = ra > 0 ? 1.0 / ra : 0.0, ib = rb > 0 ? 1.0 / rb : 0.0;
+  c.ax = (float)(xa * ia); c.ay = (float)(ya * ia); c.az = (float)(za * ia);
+  c.bx = (float)(xb * ib); c.by = (float)(yb * ib); c.bz = (float)(zb * ib);
+  // a satellite at the origin sees everything with t = 0 only if s.0 >= 0: always true
+  c.ca = ra > 0 ? (float)(m.ta * ia) : -2.0f;
+  c.cb = rb > 0 ? (float)(m.tb * ib) : -2.0f;
+}
+
+// ------------------------------------------------------------ forward
+// Grid: x = ceil(K / 256) blocks of 256 visibilities (4 wavefronts), y = pixel
+// chunks of `tiles_per_chunk` 64-pixel tiles.
+template <bool MASK, bool F64>
+__global__ __launch_bounds__(256) void k_forward(const PixPack* __restrict__ pix, const double* __restrict__ I64,
+                                                 const float4* __restrict__ cone, int64_t ntile,
+                                                 int64_t tiles_per_chunk, const double* __restrict__ uvw,
+                                                 const double* __restrict__ xyza, const double* __restrict__ xyzb,
+                                                 int64_t K, double R_moon, double orbit_r, int track, int herm,
+                                                 double* __restrict__ V) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool act = k < K;
+  const int64_t kk = act ? k : K - 1;
+  const double u = uvw[kk], v = uvw[K + kk], w = uvw[2 * K + kk];
+  const double c0 = track ? -w : 0.0;
+  VisMask mk{};
+  VisCls cl{};
+  if (MASK) {
+    make_mask_records(xyza[kk], xyza[K + kk], xyza[2 * K + kk], xyzb[kk], xyzb[K + kk], xyzb[2 * K + kk], R_moon,
+                      orbit_r, mk, cl);
+  }
+  const int64_t t0 = (int64_t)blockIdx.y * tiles_per_chunk;
+  const int64_t t1 = min(ntile, t0 + tiles_per_chunk);
+  double Sr = 0.0, Si = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    bool full = true;
+    if (MASK) {
+      bool any_v, all_v;
+      classify(cone[t], cl, any_v, all_v);
+      any_v = any_v & act;
+      all_v = all_v | !act;
+      if (__ballot(any_v) == 0) continue;
+      full = __ballot(!all_v) == 0;
+    }
+    const PixPack* __restrict__ tp = pix + t * kTile;
+    if (!F64) {
+      float ar0 = 0.f, ai0 = 0.f, ar1 = 0.f, ai1 = 0.f;
+      if (full) {
+#pragma unroll 4
+        for (int j = 0; j < kTile; j += 2) {
+          const PixPack p0 = tp[j], p1 = tp[j + 1];
+          float s0, c0f, s1, c1f;
+          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
+          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
+          ar0 = fmaf(p0.I, c0f, ar0);
+          ai0 = fmaf(p0.I, s0, ai0);
+          ar1 = fmaf(p1.I, c1f, ar1);
+          ai1 = fmaf(p1.I, s1, ai1);
+        }
+      } else {
+#pragma unroll 2
+        for (int j = 0; j < kTile; j += 2) {
+          const PixPack p0 = tp[j], p1 = tp[j + 1];
+          const float I0 = both_visible(p0.l, p0.m, p0.n, mk) ? p0.I : 0.f;
+          const float I1 = both_visible(p1.l, p1.m, p1.n, mk) ? p1.I : 0.f;
+          float s0, c0f, s1, c1f;
+          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
+          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
+          ar0 = fmaf(I0, c0f, ar0);
+          ai0 = fmaf(I0, s0, ai0);
+          ar1 = fmaf(I1, c1f, ar1);
+          ai1 = fmaf(I1, s1, ai1);
+        }
+      }
+      Sr += (double)ar0 + (double)ar1;
+      Si += (double)ai0 + (double)ai1;
+    } else {
+      const double* __restrict__ ip = I64 + t * kTile;
+      for (int j = 0; j < kTile; ++j) {
+        const PixPack p0 = tp[j];
+        double I0 = ip[j];
+        if (!full && !both_visible(p0.l, p0.m, p0.n, mk)) I0 = 0.0;
+        const double f = __builtin_amdgcn_fract(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0))));
+        double s, c;
+        sincospi(2.0 * f, &s, &c);
+        Sr = fma(I0, c, Sr);
+        Si = fma(I0, s, Si);
+      }
+    }
+  }
+  if (act) {
+    // V = sum I exp(-2 pi i ph) = sum I (cos - i sin)
+    unsafeAtomicAdd(&V[2 * k], Sr);
+    unsafeAtomicAdd(&V[2 * k + 1], -Si);
+    if (herm) {
+      unsafeAtomicAdd(&V[2 * (k + K)], Sr);
+      unsafeAtomicAdd(&V[2 * (k + K) + 1], Si);
+    }
+  }
+}
+
+// ------------------------------------------------------------ back-projection
+// Prologue: y'_k = wt_k * y_k [* exp(-2 pi i w_k)], packed with u, v, w.
+__global__ void k_bp_prologue(const double* __restrict__ uvw, const double* __restrict__ xyza,
+                              const double* __restrict__ xyzb, const double* __restrict__ y,
+                              const double* __restrict__ wt, int64_t K, int mask, int track, int f64, double R_moon,
+                              double orbit_r, VisHot* __restrict__ hot, VisHot64* __restrict__ hot64,
+                              VisMask* __restrict__ vm, VisCls* __restrict__ vc) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
+    const double u = uvw[k], v = uvw[K + k], w = uvw[2 * K + k];
+    double yr = y[2 * k], yi = y[2 * k + 1];
+    if (track) {
+      double s, c;
+      sincospi(-2.0 * (w - rint(w)), &s, &c);
+      const double nr = yr * c - yi * s, ni = yr * s + yi * c;
+      yr = nr;
+      yi = ni;
+    }
+    if (wt) {
+      yr *= wt[k];
+      yi *= wt[k];
+    }
+    if (f64) {
+      VisHot64 h;
+      h.u = u; h.v = v; h.w = w; h.yr = yr; h.yi = yi; h.pad0 = h.pad1 = h.pad2 = 0.0;
+      hot64[k] = h;
+    } else {
+      VisHot h;
+      h.u = u; h.v = v; h.w = w; h.yr = (float)yr; h.yi = (float)yi;
+      hot[k] = h;
+    }
+    if (mask) {
+      VisMask m;
+      VisCls c;
+      make_mask_records(xyza[k], xyza[K + k], xyza[2 * K + k], xyzb[k], xyzb[K + k], xyzb[2 * K + k], R_moon,
+                        orbit_r, m, c);
+      vm[k] = m;
+      vc[k] = c;
+    }
+  }
+}
+
+// Grid: x = ceil(nbtile / 4) blocks of 4 wavefronts (one kBpTile-pixel tile per
+// wavefront), y = visibility chunks of `k_per_chunk` rows.
+template <bool MASK, bool F64>
+__global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__ pix, const float4* __restrict__ bcone,
+                                                     int64_t nbtile, const VisHot* __restrict__ hot,
+                                                     const VisHot64* __restrict__ hot64,
+                                                     const VisMask* __restrict__ vm, const VisCls* __restrict__ vc,
+                                                     int64_t K, int64_t k_per_chunk, double herm,
+                                                     double* __restrict__ x) {
+  constexpr int PPL = kBpPixPerLane;
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= nbtile) return;  // wave-uniform
+  double l[PPL], m[PPL], n[PPL];
+  int32_t idx[PPL];
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    const PixPack q = pix[tile * kBpTile + i * 64 + lane];
+    l[i] = q.l;
+    m[i] = q.m;
+    n[i] = q.n;
+    idx[i] = q.idx;
+  }
+  float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
+  if (MASK) cn = bcone[tile];
+  const int64_t k0 = (int64_t)blockIdx.y * k_per_chunk;
+  const int64_t k1 = min(K, k0 + k_per_chunk);
+  double acc[PPL];
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) acc[i] = 0.0;
+  for (int64_t kb = k0; kb < k1; kb += 64) {
+    const int nk = (int)min((int64_t)64, k1 - kb);
+    uint64_t anym, fullm;
+    if (MASK) {
+      bool any_v = false, all_v = false;
+      if (lane < nk) classify(cn, vc[kb + lane], any_v, all_v);
+      anym = __ballot(any_v);
+      fullm = __ballot(all_v);
+    } else {
+      anym = nk == 64 ? ~0ull : ((1ull << nk) - 1ull);
+      fullm = anym;
+    }
+    if (!F64) {
+      float a[PPL];
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) a[i] = 0.f;
+      while (anym) {
+        const int j = __builtin_ctzll(anym);
+        anym &= anym - 1ull;
+        const VisHot h = hot[kb + j];
+        if (!MASK || ((fullm >> j) & 1ull)) {
+#pragma unroll
+          for (int i = 0; i < PPL; ++i) {
+            float s, c;
+            sincos_rev(rev_fp32(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i]))), s, c);
+            a[i] = fmaf(h.yr, c, fmaf(-h.yi, s, a[i]));
+          }
+        } else {
+          const VisMask q = vm[kb + j];
+#pragma unroll
+          for (int i = 0; i < PPL; ++i) {
+            float s, c;
+            sincos_rev(rev_fp32(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i]))), s, c);
+            const float t = fmaf(h.yr, c, -h.yi * s);
+            a[i] += both_visible(l[i], m[i], n[i], q) ? t : 0.f;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) acc[i] += (double)a[i];
+    } else {
+      while (anym) {
+        const int j = __builtin_ctzll(anym);
+        anym &= anym - 1ull;
+        const VisHot64 h = hot64[kb + j];
+        const bool full = !MASK || ((fullm >> j) & 1ull);
+        VisMask q{};
+        if (!full) q = vm[kb + j];
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) {
+          const double f = __builtin_amdgcn_fract(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i])));
+          double s, c;
+          sincospi(2.0 * f, &s, &c);
+          const double t = fma(h.yr, c, -h.yi * s);
+          if (full || both_visible(l[i], m[i], n[i], q)) acc[i] += t;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PPL; ++i)
+    if (idx[i] >= 0) unsafeAtomicAdd(&x[idx[i]], herm * acc[i]);
+}
+
+// ------------------------------------------------------------ visible-pair count
+__global__ __launch_bounds__(256) void k_count_visible(const PixPack* __restrict__ pix,
+                                                       const float4* __restrict__ cone,
+                                                       const int32_t* __restrict__ tile_valid, int64_t ntile,
+                                                       int64_t tiles_per_chunk, const double* __restrict__ xyza,
+                                                       const double* __restrict__ xyzb, int64_t K, double R_moon,
+                                                       double orbit_r, unsigned long long* __restrict__ count) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool act = k < K;
+  const int64_t kk = act ? k : K - 1;
+  VisMask mk;
+  VisCls cl;
+  make_mask_records(xyza[kk], xyza[K + kk], xyza[2 * K + kk], xyzb[kk], xyzb[K + kk], xyzb[2 * K + kk], R_moon,
+                    orbit_r, mk, cl);
+  const int64_t t0 = (int64_t)blockIdx.y * tiles_per_chunk;
+  const int64_t t1 = min(ntile, t0 + tiles_per_chunk);
+  unsigned long long c = 0;
+  for (int64_t t = t0; t < t1; ++t) {
+    bool any_v, all_v;
+    classify(cone[t], cl, any_v, all_v);
+    if (!any_v) continue;
+    if (all_v) {
+      c += (unsigned long long)tile_valid[t];
+      continue;
+    }
+    const PixPack* tp = pix + t * kTile;
+    for (int j = 0; j < kTile; ++j) {
+      const PixPack q = tp[j];
+      c += (q.idx >= 0 && both_visible(q.l, q.m, q.n, mk)) ? 1ull : 0ull;
+    }
+  }
+  if (act && c) atomicAdd(count, c);
+}
+
+// ------------------------------------------------------------ launch helpers
+static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Split the second grid dimension so that the launch has >= ~32 wavefronts per
+// SIMD (256 CUs x 4 SIMDs) while each chunk keeps >= min_chunk units of work.
+static int64_t choose_chunks(int64_t waves_x, int64_t units, int64_t min_chunk) {
+  const int64_t target_waves = 256 * 4 * 32;
+  int64_t ch = cdiv(target_waves, waves_x > 0 ? waves_x : 1);
+  const int64_t max_ch = units / min_chunk > 0 ? units / min_chunk : 1;
+  if (ch > max_ch) ch = max_ch;
+  if (ch > 65535) ch = 65535;
+  return ch < 1 ? 1 : ch;
+}
+
+static int validate_geom(const viss_mask_geometry* g, int flags) {
+  if (flags & VISS_MASK) {
+    VISS_REQUIRE(g != nullptr, "VISS_MASK needs a viss_mask_geometry");
+    VISS_REQUIRE(g->R_moon >= 0.0, "R_moon must be >= 0");
+    VISS_REQUIRE(g->orbit_radius <= 0.0 || g->orbit_radius > g->R_moon,
+                 "orbit_radius must exceed R_moon when given (fixed-cut mode)");
+  }
+  return VISS_OK;
+}
+
+static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, const double* xyza,
+                            const double* xyzb, int64_t K, const viss_mask_geometry* geom, int flags, double* V,
+                            hipStream_t s) {
+  VISS_REQUIRE(K >= 0, "viss_forward: negative K");
+  if (K == 0) return VISS_OK;
+  VISS_REQUIRE(K <= ((int64_t)1 << 40), "viss_forward: K too large");
+  VISS_REQUIRE(I && uvw && V, "viss_forward: I/uvw/V must not be NULL");
+  const bool mask = flags & VISS_MASK;
+  VISS_REQUIRE(!mask || (xyza && xyzb), "viss_forward: VISS_MASK needs xyz_a and xyz_b");
+  int rc = validate_geom(geom, flags);
+  if (rc != VISS_OK) return rc;
+  const bool f64 = flags & VISS_PREC_FP64;
+  rc = sky_set_brightness(sky, I, f64, s);
+  if (rc != VISS_OK) return rc;
+  const double R = mask ? geom->R_moon : 0.0, orb = mask ? geom->orbit_radius : 0.0;
+  const int64_t bx = cdiv(K, 256);
+  const int64_t chunks = choose_chunks(bx * 4, sky->ntile, 64);
+  const int64_t tpc = cdiv(sky->ntile, chunks);
+  VISS_REQUIRE(bx <= INT32_MAX, "viss_forward: K too large for one launch");
+  dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
+  const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
+#define VISS_FWD(M, F)                                                                                          \
+  k_forward<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->I64, sky->cone, sky->ntile, tpc, uvw, xyza, xyzb, K, R, orb, \
+                                       track, herm, V)
+  if (mask) {
+    if (f64) VISS_FWD(true, true); else VISS_FWD(true, false);
+  } else {
+    if (f64) VISS_FWD(false, true); else VISS_FWD(false, false);
+  }
+#undef VISS_FWD
+  VISS_LAUNCHED(k_forward);
+  return VISS_OK;
+}
+
+static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb,
+                                const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
+                                int flags, double* x, hipStream_t s) {
+  VISS_REQUIRE(K >= 0, "viss_backproject: negative K");
+  if (K == 0) return VISS_OK;
+  VISS_REQUIRE(K <= ((int64_t)1 << 40), "viss_backproject: K too large");
+  VISS_REQUIRE(uvw && y && x, "viss_backproject: uvw/y/x must not be NULL");
+  const bool mask = flags & VISS_MASK;
+  VISS_REQUIRE(!mask || (xyza && xyzb), "viss_backproject: VISS_MASK needs xyz_a and xyz_b");
+  int rc = validate_geom(geom, flags);
+  if (rc != VISS_OK) return rc;
+  const bool f64 = flags & VISS_PREC_FP64;
+  const size_t hot_b = (size_t)K * (f64 ? sizeof(VisHot64) : sizeof(VisHot));
+  const size_t msk_b = mask ? (size_t)K * (sizeof(VisMask) + sizeof(VisCls)) : 0;
+  rc = sky->ws.reserve(hot_b + msk_b + 256, s);
+  if (rc != VISS_OK) return rc;
+  char* base = (char*)sky->ws.p;
+  VisHot* hot = f64 ? nullptr : (VisHot*)base;
+  VisHot64* hot64 = f64 ? (VisHot64*)base : nullptr;
+  VisMask* vm = mask ? (VisMask*)(base + hot_b) : nullptr;
+  VisCls* vc = mask ? (VisCls*)(base + hot_b + (size_t)K * sizeof(VisMask)) : nullptr;
+  const double R = mask ? geom->R_moon : 0.0, orb = mask ? geom->orbit_radius : 0.0;
+  {
+    int64_t g = cdiv(K, 256);
+    if (g > 65536) g = 65536;
+    k_bp_prologue<<<(unsigned)g, 256, 0, s>>>(uvw, xyza, xyzb, y, wt, K, mask, (flags & VISS_PHASE_TRACK) ? 1 : 0,
+                                              f64 ? 1 : 0, R, orb, hot, hot64, vm, vc);
+    VISS_LAUNCHED(k_bp_prologue);
+  }
+  const int64_t bx = cdiv(sky->nbtile, 4);
+  const int64_t chunks = choose_chunks(sky->nbtile, cdiv(K, 64), 16);
+  int64_t kpc = cdiv(cdiv(K, chunks), 64) * 64;
+  dim3 grid((unsigned)bx, (unsigned)cdiv(K, kpc));
+  const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
+#define VISS_BP(M, F) \
+  k_backproject<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, kpc, herm, x)
+  if (mask) {
+    if (f64) VISS_BP(true, true); else VISS_BP(true, false);
+  } else {
+    if (f64) VISS_BP(false, true); else VISS_BP(false, false);
+  }
+#undef VISS_BP
+  VISS_LAUNCHED(k_backproject);
+  return VISS_OK;
+}
+
+}  // namespace viss
+
+using namespace viss;
+
+extern "C" {
+
+int viss_sky_forward(viss_sky* sky, const double* I, const double* uvw, const double* xyz_a, const double* xyz_b,
+                     int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_forward: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  return sky_forward_impl(sky, I, uvw, xyz_a, xyz_b, K, geom, flags, V, as_stream(stream));
+}
+
+int viss_sky_backproject(viss_sky* sky, const double* uvw, const double* xyz_a, const double* xyz_b,
+                         const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
+                         double* x, void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_backproject: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  return sky_backproject_impl(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, as_stream(stream));
+}
+
+int viss_forward(const double* lmn, const double* I, int64_t P, const double* uvw, const double* xyz_a,
+                 const double* xyz_b, int64_t K, const viss_mask_geometry* geom, int flags, double* V,
+                 void* stream) {
+  VISS_REQUIRE(P >= 0 && K >= 0, "viss_forward: negative size");
+  if (P == 0 || K == 0) return VISS_OK;
+  VISS_REQUIRE(lmn != nullptr, "viss_forward: lmn is NULL");
+  viss_sky* sky = nullptr;
+  int rc = viss_sky_create(lmn, P, &sky, stream);
+  if (rc != VISS_OK) return rc;
+  rc = sky_forward_impl(sky, I, uvw, xyz_a, xyz_b, K, geom, flags, V, as_stream(stream));
+  viss_sky_destroy(sky);
+  return rc;
+}
+
+int viss_backproject(const double* lmn, int64_t P, const double* uvw, const double* xyz_a, const double* xyz_b,
+                     const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
+                     double* x, void* stream) {
+  VISS_REQUIRE(P >= 0 && K >= 0, "viss_backproject: negative size");
+  if (P == 0 || K == 0) return VISS_OK;
+  VISS_REQUIRE(lmn != nullptr, "viss_backproject: lmn is NULL");
+  viss_sky* sky = nullptr;
+  int rc = viss_sky_create(lmn, P, &sky, stream);
+  if (rc != VISS_OK) return rc;
+  rc = sky_backproject_impl(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, as_stream(stream));
+  viss_sky_destroy(sky);
+  return rc;
+}
+
+int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,
+                           const viss_mask_geometry* geom, int64_t* count, void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_count_visible: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE(K >= 0, "viss_sky_count_visible: negative K");
+  if (K == 0) return VISS_OK;
+  VISS_REQUIRE(xyz_a && xyz_b && count && geom, "viss_sky_count_visible: NULL argument");
+  rc = validate_geom(geom, VISS_MASK);
+  if (rc != VISS_OK) return rc;
+  const int64_t bx = cdiv(K, 256);
+  const int64_t chunks = choose_chunks(bx * 4, sky->ntile, 64);
+  const int64_t tpc = cdiv(sky->ntile, chunks);
+  dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
+  k_count_visible<<<grid, 256, 0, as_stream(stream)>>>(sky->pix, sky->cone, sky->tile_valid, sky->ntile, tpc, xyz_a,
+                                                       xyz_b, K, geom->R_moon, geom->orbit_radius,
+                                                       (unsigned long long*)count);
+  VISS_LAUNCHED(k_count_visible);
+  return VISS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+// HEALPix pix2ang (NEST and RING) and the direction pre-pass.
+//
+// Replaces mk_pix2xy + f_pix2and_nest (pix2ang.cu:61-160).  The reference
+// kernel writes twelve [npix] scratch arrays to HBM, indexes its 1-based
+// MATLAB tables with +1 (so every theta is NaN) and has no bounds guard.  Here
+// each thread holds its pixel's integers in registers and writes only the
+// outputs (16 B/pixel for theta, phi; +8 with ring/iphi): an HBM-store-bound
+// elementwise kernel.  The Morton de-interleave is done with bit masks, so
+// the 1024-entry pix2x/pix2y lookup table is not needed.
+//
+// Bit-exactness: z = (3n^2 - nr^2) / (3n^2) and phi = (pi/2) t / nr are single
+// correctly rounded FP64 operations on exact integers, identical to the host
+// oracle.  theta = acos(z) uses the device library's acos (within 1 ulp of
+// the host's).
+#include "viss_internal.h"
+
+namespace viss {
+
+__device__ inline int64_t compact_even_bits(uint64_t v) {
+  v &= 0x5555555555555555ull;
+  v = (v | (v >> 1)) & 0x3333333333333333ull;
+  v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
+  v = (v | (v >> 16)) & 0x00000000ffffffffull;
+  return (int64_t)v;
+}
+
+__device__ inline int64_t isqrt64(int64_t x) {
+  int64_t s = (int64_t)sqrt((double)x);
+  while (s * s > x) --s;
+  while ((s + 1) * (s + 1) <= x) ++s;
+  return s;
+}
+
+struct RingPos {
+  int64_t jr, jp, nr, kshift;
+};
+
+// NEST: face / in-face (ix, iy) -> ring jr and in-ring jp (pix2ang.cu:99-157,
+// 0-based tables jrll/jpll of pix2ang.cu:183-184).
+__device__ inline RingPos nest_pos(int64_t ip, int64_t nside) {
+  constexpr int8_t jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
+  constexpr int8_t jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
+  const int64_t npface = nside * nside;
+  const int64_t face = ip / npface, ipf = ip - face * npface;
+  const int64_t ix = compact_even_bits((uint64_t)ipf), iy = compact_even_bits((uint64_t)ipf >> 1);
+  RingPos r;
+  r.jr = jrll[face] * nside - (ix + iy) - 1;
+  if (r.jr < nside) {
+    r.nr = r.jr;
+    r.kshift = 0;
+  } else if (r.jr <= 3 * nside) {
+    r.nr = nside;
+    r.kshift = (r.jr - nside) & 1;
+  } else {
+    r.nr = 4 * nside - r.jr;
+    r.kshift = 0;
+  }
+  int64_t jp = (jpll[face] * r.nr + (ix - iy) + 1 + r.kshift) / 2;
+  if (jp > 4 * nside) jp -= 4 * nside;
+  if (jp < 1) jp += 4 * nside;
+  r.jp = jp;
+  return r;
+}
+
+// RING: Gorski et al. 2005 sec. 4 (polar caps / equatorial belt).
+__device__ inline RingPos ring_pos(int64_t p, int64_t nside) {
+  const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1), nl4 = 4 * nside;
+  RingPos r;
+  r.kshift = 0;
+  if (p < ncap) {
+    const int64_t i = (1 + isqrt64(1 + 2 * p)) / 2;
+    r.jr = i;
+    r.nr = i;
+    r.jp = p + 1 - 2 * i * (i - 1);
+  } else if (p < npix - ncap) {
+    const int64_t q = p - ncap;
+    r.jr = q / nl4 + nside;
+    r.jp = q % nl4 + 1;
+    r.nr = nside;
+    r.kshift = (r.jr - nside) & 1;
+  } else {
+    const int64_t q = npix - p;
+    const int64_t i = (1 + isqrt64(2 * q - 1)) / 2;
+    r.jr = nl4 - i;
+    r.nr = i;
+    r.jp = 4 * i + 1 - (q - 2 * i * (i - 1));
+  }
+  return r;
+}
+
+#pragma clang fp contract(off)
+__device__ inline void ring_angles(const RingPos& r, int64_t nside, double& z, double& phi) {
+  const double three_ns2 = 3.0 * (double)(nside * nside);
+  if (r.jr < nside) {
+    z = (three_ns2 - (double)(r.nr * r.nr)) / three_ns2;
+  } else if (r.jr <= 3 * nside) {
+    z = (double)(2 * (2 * nside - r.jr)) / (double)(3 * nside);
+  } else {
+    z = -((three_ns2 - (double)(r.nr * r.nr)) / three_ns2);
+  }
+  phi = (kPi / 2.0) * ((double)r.jp - 0.5 * (double)(r.kshift + 1)) / (double)r.nr;
+}
+
+// lmn (optional) is computed from theta, phi exactly as the pre-pass does
+// (vissgen3d_noblockage.cu:58-70): l = sin t cos p, m = -sin t sin p, n = cos t.
+__global__ void k_pix2ang(int64_t nside, int nest, int64_t npix, double* __restrict__ theta, double* __restrict__ phi,
+                          int32_t* __restrict__ ring, int32_t* __restrict__ iphi) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+    const RingPos r = nest ? nest_pos(p, nside) : ring_pos(p, nside);
+    double z, ph;
+    ring_angles(r, nside, z, ph);
+    theta[p] = acos(z);
+    phi[p] = ph;
+    if (ring) ring[p] = (int32_t)r.jr;
+    if (iphi) iphi[p] = (int32_t)r.jp;
+  }
+}
+
+__global__ void k_pixel_dirs(const double* __restrict__ theta, const double* __restrict__ phi,
+                             const double* __restrict__ B, int64_t P, double scale, double* __restrict__ lmn,
+                             double* __restrict__ I) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P; p += (int64_t)gridDim.x * blockDim.x) {
+    double st, ct, sp, cp;
+    sincos(theta[p], &st, &ct);
+    sincos(phi[p], &sp, &cp);
+    lmn[p] = st * cp;
+    lmn[P + p] = -(st * sp);
+    lmn[2 * P + p] = ct;
+    if (I) I[p] = B[p] * scale;
+  }
+}
+#pragma clang fp contract(on)
+
+static unsigned grid_1d(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  const int64_t cap = 256 * 64;  // 64 blocks of 256 per CU: enough to fill 256 CUs, grid-stride beyond
+  if (g > cap) g = cap;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace viss
+
+using namespace viss;
+
+extern "C" int viss_pix2ang(int64_t nside, int scheme, double* theta, double* phi, int32_t* ring, int32_t* iphi,
+                            void* stream) {
+  VISS_REQUIRE(nside >= 1 && nside <= 8192, "viss_pix2ang: nside %lld outside [1, 8192]", (long long)nside);
+  VISS_REQUIRE(scheme == VISS_SCHEME_RING || scheme == VISS_SCHEME_NEST, "viss_pix2ang: unknown scheme %d", scheme);
+  VISS_REQUIRE(theta && phi, "viss_pix2ang: theta/phi must not be NULL");
+  VISS_REQUIRE(scheme == VISS_SCHEME_RING || (nside & (nside - 1)) == 0,
+               "viss_pix2ang: NEST needs a power-of-two nside (got %lld)", (long long)nside);
+  const int64_t npix = 12 * nside * nside;
+  k_pix2ang<<<grid_1d(npix), 256, 0, as_stream(stream)>>>(nside, scheme == VISS_SCHEME_NEST, npix, theta, phi, ring,
+                                                           iphi);
+  VISS_LAUNCHED(k_pix2ang);
+  return VISS_OK;
+}
+
+extern "C" int viss_pixel_dirs(const double* theta, const double* phi, const double* B, int64_t P, double scale,
+                               double* lmn, double* I, void* stream) {
+  VISS_REQUIRE(P >= 0, "viss_pixel_dirs: negative P");
+  if (P == 0) return VISS_OK;
+  VISS_REQUIRE(theta && phi && lmn, "viss_pixel_dirs: theta/phi/lmn must not be NULL");
+  VISS_REQUIRE((B == nullptr) == (I == nullptr), "viss_pixel_dirs: B and I must both be given or both NULL");
+  k_pixel_dirs<<<grid_1d(P), 256, 0, as_stream(stream)>>>(theta, phi, B, P, scale, lmn, I);
+  VISS_LAUNCHED(k_pixel_dirs);
+  return VISS_OK;
+}

@@ -1,0 +1,201 @@
+"""Torch-tensor front end of the HIP library.
+
+Mirrors the reference's kernels as operators on device tensors (the
+reference's names in brackets):
+
+* :func:`pix2ang`      [mk_pix2xy + f_pix2and_nest, pix2ang.cu:61-160]
+* :func:`pixel_dirs`   [healpix_moonback_pre, vissgen3d_noblockage.cu:52-73]
+* :class:`Sky`.forward [healpix_moonback_viss, vissgen3d_noblockage.cu:76-102]
+* :class:`Sky`.backproject [computeC, vissgen3d_noblockage.cu:167-182 and
+  vissgen3d_blockage.cu:167-192; the adjoint when ``wt`` is None]
+* :func:`weights`      [ceilAndScale ... viss_gamma_trans, vissgen3d_noblockage.cu:105-164]
+
+Torch only supplies device memory and the current stream; every computation is
+a call into libviss_hip.so.  Tensors must live on a HIP device; inputs are
+made contiguous in the dtype the C ABI declares.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _lib
+from ._lib import MaskGeometry, check
+
+REF_R_MOON = 1737.1e3
+REF_ORBIT_RADIUS = 1737.1e3 + 300e3
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _dev(t: torch.Tensor, dtype) -> torch.Tensor:
+    if not t.is_cuda:
+        raise ValueError("viss ops take device tensors")
+    return t.to(dtype).contiguous()
+
+
+def _flags(mask=False, phase_track=False, fp64=False, hermitian=False):
+    return ((_lib.MASK if mask else 0) | (_lib.PHASE_TRACK if phase_track else 0) |
+            (_lib.PREC_FP64 if fp64 else 0) | (_lib.HERMITIAN if hermitian else 0))
+
+
+def _geom(R_moon: float, orbit_radius: float):
+    return ctypes.byref(MaskGeometry(float(R_moon), float(orbit_radius)))
+
+
+def pix2ang(nside: int, nest: bool = True, device="cuda", with_ring: bool = True):
+    """theta, phi (float64 [12 nside^2]) and, optionally, 1-based ring / in-ring index (int32)."""
+    L = _lib.load()
+    P = 12 * nside * nside
+    th = torch.empty(P, dtype=torch.float64, device=device)
+    ph = torch.empty(P, dtype=torch.float64, device=device)
+    ring = torch.empty(P, dtype=torch.int32, device=device) if with_ring else None
+    iphi = torch.empty(P, dtype=torch.int32, device=device) if with_ring else None
+    check("viss_pix2ang", L.viss_pix2ang(nside, _lib.SCHEME_NEST if nest else _lib.SCHEME_RING, _p(th), _p(ph),
+                                         _p(ring), _p(iphi), _stream()))
+    return (th, ph, ring, iphi) if with_ring else (th, ph)
+
+
+def pixel_dirs(theta: torch.Tensor, phi: torch.Tensor, B: Optional[torch.Tensor] = None,
+               scale: Optional[float] = None):
+    """lmn (float64 [3, P]) and I = B * scale (default scale 4 pi / P)."""
+    L = _lib.load()
+    theta = _dev(theta, torch.float64)
+    phi = _dev(phi, torch.float64)
+    P = theta.numel()
+    lmn = torch.empty(3, P, dtype=torch.float64, device=theta.device)
+    I = None
+    if B is not None:
+        B = _dev(B, torch.float64)
+        I = torch.empty(P, dtype=torch.float64, device=theta.device)
+        if scale is None:
+            scale = 4.0 * torch.pi / P
+    check("viss_pixel_dirs", L.viss_pixel_dirs(_p(theta), _p(phi), _p(B), P, float(scale or 0.0), _p(lmn), _p(I),
+                                               _stream()))
+    return lmn, I
+
+
+class Sky:
+    """A pixel set sorted into compact tiles on the device (``viss_sky``)."""
+
+    def __init__(self, lmn: torch.Tensor):
+        L = _lib.load()
+        self.lmn = _dev(lmn, torch.float64)
+        if self.lmn.dim() != 2 or self.lmn.shape[0] != 3:
+            raise ValueError("lmn must be [3, P]")
+        self.P = self.lmn.shape[1]
+        self.device = self.lmn.device
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check("viss_sky_create", L.viss_sky_create(_p(self.lmn), self.P, ctypes.byref(h), _stream()))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            torch.cuda.synchronize(self.device)
+            _lib.load().viss_sky_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _rows(self, uvw, xyz_a, xyz_b, mask):
+        uvw = _dev(uvw, torch.float64)
+        if uvw.dim() != 2 or uvw.shape[0] != 3:
+            raise ValueError("uvw must be [3, K]")
+        K = uvw.shape[1]
+        xa = xb = None
+        if mask:
+            if xyz_a is None or xyz_b is None:
+                raise ValueError("mask=True needs xyz_a and xyz_b")
+            xa = _dev(xyz_a, torch.float64)
+            xb = _dev(xyz_b, torch.float64)
+            if tuple(xa.shape) != (3, K) or tuple(xb.shape) != (3, K):
+                raise ValueError("xyz_a / xyz_b must be [3, K]")
+        return uvw, xa, xb, K
+
+    def forward(self, I: torch.Tensor, uvw: torch.Tensor, xyz_a=None, xyz_b=None, *, mask: bool = False,
+                R_moon: float = REF_R_MOON, orbit_radius: float = 0.0, phase_track: bool = False,
+                hermitian: bool = False, fp64: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """V (complex128 [K] or [2K]).  Accumulates into ``out`` when given."""
+        L = _lib.load()
+        I = _dev(I, torch.float64)
+        if I.numel() != self.P:
+            raise ValueError("I must have P entries")
+        uvw, xa, xb, K = self._rows(uvw, xyz_a, xyz_b, mask)
+        rows = 2 * K if hermitian else K
+        if out is None:
+            out = torch.zeros(rows, dtype=torch.complex128, device=self.device)
+        elif out.dtype != torch.complex128 or out.numel() != rows or not out.is_contiguous():
+            raise ValueError("out must be a contiguous complex128 tensor of K (2K) rows")
+        check("viss_sky_forward", L.viss_sky_forward(self._h, _p(I), _p(uvw), _p(xa), _p(xb), K,
+                                                     _geom(R_moon, orbit_radius),
+                                                     _flags(mask, phase_track, fp64, hermitian), _p(out), _stream()))
+        return out
+
+    def backproject(self, uvw: torch.Tensor, y: torch.Tensor, wt: Optional[torch.Tensor] = None, xyz_a=None,
+                    xyz_b=None, *, mask: bool = False, R_moon: float = REF_R_MOON, orbit_radius: float = 0.0,
+                    phase_track: bool = False, hermitian: bool = False, fp64: bool = False,
+                    out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """x (float64 [P]) = herm * Re sum_k [mask] wt_k y'_k exp(+2 pi i b_k . s_p)."""
+        L = _lib.load()
+        uvw, xa, xb, K = self._rows(uvw, xyz_a, xyz_b, mask)
+        y = _dev(y, torch.complex128)
+        if y.numel() != K:
+            raise ValueError("y must have K entries")
+        w = None if wt is None else _dev(wt, torch.float64)
+        if out is None:
+            out = torch.zeros(self.P, dtype=torch.float64, device=self.device)
+        check("viss_sky_backproject", L.viss_sky_backproject(self._h, _p(uvw), _p(xa), _p(xb), _p(y), _p(w), K,
+                                                             _geom(R_moon, orbit_radius),
+                                                             _flags(mask, phase_track, fp64, hermitian), _p(out),
+                                                             _stream()))
+        return out
+
+    def count_visible(self, xyz_a, xyz_b, *, R_moon: float = REF_R_MOON, orbit_radius: float = 0.0) -> torch.Tensor:
+        L = _lib.load()
+        xa = _dev(xyz_a, torch.float64)
+        xb = _dev(xyz_b, torch.float64)
+        K = xa.shape[1]
+        cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
+        check("viss_sky_count_visible", L.viss_sky_count_visible(self._h, _p(xa), _p(xb), K,
+                                                                 _geom(R_moon, orbit_radius), _p(cnt), _stream()))
+        return cnt
+
+
+def weights(bll: torch.Tensor, w: torch.Tensor, nr: int, blockage_shift: bool = False, want_dg: bool = True):
+    """Dict with gs (int32), mb (uint32 as int64 view... stored int32), dcf, dg, wt (float64), n_out_of_range."""
+    L = _lib.load()
+    bll = _dev(bll, torch.float32)
+    w = _dev(w, torch.float64)
+    K = bll.numel()
+    dev = bll.device
+    gs = torch.empty(K, dtype=torch.int32, device=dev)
+    mb = torch.zeros(nr, dtype=torch.int32, device=dev)  # uint32 counts (< 2^31 in practice)
+    dcf = torch.empty(nr + 1, dtype=torch.float64, device=dev)
+    dg = torch.empty(K, dtype=torch.float64, device=dev) if want_dg else None
+    wt = torch.empty(K, dtype=torch.float64, device=dev)
+    bad = torch.zeros(1, dtype=torch.int64, device=dev)
+    check("viss_weights", L.viss_weights(_p(bll), _p(w), K, int(nr), _lib.DCF_BLOCKAGE_SHIFT if blockage_shift else 0,
+                                         _p(gs), _p(mb), _p(dcf), _p(dg), _p(wt), _p(bad), _stream()))
+    return {"gs": gs, "mb": mb, "dcf": dcf, "dg": dg, "wt": wt, "n_out_of_range": bad}
+
+
+def forward(lmn, I, uvw, xyz_a=None, xyz_b=None, **kw):
+    """One-shot forward (builds a temporary Sky)."""
+    return Sky(lmn).forward(I, uvw, xyz_a, xyz_b, **kw)
+
+
+def backproject(lmn, uvw, y, wt=None, xyz_a=None, xyz_b=None, **kw):
+    return Sky(lmn).backproject(uvw, y, wt, xyz_a, xyz_b, **kw)
