@@ -1,0 +1,291 @@
+"""Throughput of the lunar-orbit interferometer hot path on one MI355X.
+
+One step = the mixed-precision forward (3D visibilities with the occultation
+mask) followed by its adjoint (masked back-projection) over the whole
+"precision" workload: HEALPix RING nside=256 (786,432 pixels), 10 MHz,
+10 satellites (45 baselines) x 20,000 snapshots = 900,000 visibilities,
+sigma = 50 km offsets, orbit normal rotating about the spin axis.  Inputs are
+synthetic (seeded) and resident in HBM before timing starts.
+
+value = evaluated (visibility, pixel) pairs per second over forward + adjoint,
+where evaluated pairs are the pairs the occultation mask lets through (the
+exact count of m_kp = 1, from viss_sky_count_visible), counted once per pass.
+
+Printed extras: the all-FP64 GPU direct sum on the same work, relative L2 of
+the mixed result against it and against the FP64 CPU oracle on a sample, the
+FP64 CPU oracle's own throughput on that sample (OpenMP on the host cores),
+and HBM rates of the pix2ang and weights passes.
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload precision-nside256]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "vis·pixel pairs/s fwd+adjoint, mixed vs FP64 GPU, rel L2 ≤1e-5, nside=256"
+# MI355X published vector peaks (dense, non-matrix), TFLOP/s, and HBM3E bandwidth.
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+# Frozen flop convention per evaluated masked pair: 6 FP64 (3 FMA phase + mod-1
+# reduction) + 34 FP32 (sin + cos counted 20, complex MAC 4, two mask dots 10).
+PAIR_FP64 = 6
+PAIR_FP32 = 34
+PAIR_T_PEAK = PAIR_FP32 / PEAK_FP32 + PAIR_FP64 / PEAK_FP64  # s per pair at peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="precision-nside256")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the oracle sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 GPU comparison")
+    ap.add_argument("--no-extras", action="store_true", help="skip the pix2ang / weights HBM measurements")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from viss_optimize_3d_amd import geometry as geo
+    from viss_optimize_3d_amd import ops
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    wl = geo.WORKLOADS[args.workload]
+    log(f"[bench] generating {wl.name}: nside={wl.nside} K={wl.n_vis}")
+    inp = geo.generate(wl)
+    K = inp.uvw.shape[1]
+
+    # pixel directions on the GPU: RING pix2ang -> (l, m, n), I = B * 4 pi / P
+    th, ph = ops.pix2ang(wl.nside, nest=False, with_ring=False)
+    B = torch.from_numpy(inp.B).to(dev)
+    lmn, I = ops.pixel_dirs(th, ph, B)
+    uvw = torch.from_numpy(inp.uvw).to(dev)
+    xa = torch.from_numpy(inp.xyz_a).to(dev)
+    xb = torch.from_numpy(inp.xyz_b).to(dev)
+    R = geo.R_MOON
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sky = ops.Sky(lmn)
+    torch.cuda.synchronize()
+    sky_ms = (time.perf_counter() - t0) * 1e3
+    visible = int(sky.count_visible(xa, xb, R_moon=R).item())
+    dense = K * sky.P
+    log(f"[bench] sky built in {sky_ms:.1f} ms; visible pairs {visible} of {dense} ({visible / dense:.3f})")
+
+    V = torch.zeros(K, dtype=torch.complex128, device=dev)
+    x = torch.zeros(sky.P, dtype=torch.float64, device=dev)
+    y = torch.empty(K, dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step(fp64=False, ev=None):
+        V.zero_()
+        x.zero_()
+        if ev:
+            ev[0].record(stream)
+        sky.forward(I, uvw, xa, xb, mask=True, R_moon=R, fp64=fp64, out=V)
+        if ev:
+            ev[1].record(stream)
+        y.copy_(V)
+        if ev:
+            ev[2].record(stream)
+        sky.backproject(uvw, y, None, xa, xb, mask=True, R_moon=R, fp64=fp64, out=x)
+        if ev:
+            ev[3].record(stream)
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(ev=evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = [e[0].elapsed_time(e[1]) for e in evs]
+    adj_ms = [e[2].elapsed_time(e[3]) for e in evs]
+    ms_per_step = wall * 1e3 / args.steps
+    pairs_per_step = 2 * visible
+    value = pairs_per_step / (wall / args.steps)
+    log(f"[bench] {args.steps} steps: {ms_per_step:.1f} ms/step; fwd {np.mean(fwd_ms):.1f} ms adj {np.mean(adj_ms):.1f} ms")
+    V_mixed = V.clone()
+    x_mixed = x.clone()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64-phase+f32-sincos", "data": "synthetic",
+        "config": {"workload": wl.name, "nside": wl.nside, "npix": sky.P, "n_vis": K, "freq_hz": wl.freq_hz,
+                   "n_sat": wl.n_sat, "sigma_m": wl.sigma_m, "n_snap": wl.n_snap, "mask": True,
+                   "scheme": "RING", "seed": wl.seed},
+    }
+    fwd_avg, adj_avg = float(np.mean(fwd_ms)), float(np.mean(adj_ms))
+    dom, dom_ms = ("k_forward", fwd_avg) if fwd_avg >= adj_avg else ("k_backproject", adj_avg)
+    flops = visible * (PAIR_FP32 + PAIR_FP64)
+    achieved = flops / (dom_ms * 1e-3)
+    peak = (PAIR_FP32 + PAIR_FP64) / PAIR_T_PEAK
+    out["roofline"] = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                       "flops_per_pair": {"fp32": PAIR_FP32, "fp64": PAIR_FP64}, "pairs_per_launch": visible}
+    out["detail"] = {
+        "visible_pairs_per_pass": visible, "dense_pairs_per_pass": dense, "visible_fraction": visible / dense,
+        "fwd_ms": fwd_avg, "adj_ms": adj_avg, "fwd_pairs_per_s": visible / (fwd_avg * 1e-3),
+        "adj_pairs_per_s": visible / (adj_avg * 1e-3), "dense_equiv_pairs_per_s": 2 * dense / (wall / args.steps),
+        "sky_build_ms": sky_ms,
+    }
+
+    if not args.no_fp64:
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        step(fp64=True)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(fp64=True, ev=e)
+        torch.cuda.synchronize()
+        w64 = time.perf_counter() - t0
+        v64 = pairs_per_step / w64
+        rel_v = float(torch.linalg.vector_norm(V_mixed - V) / torch.linalg.vector_norm(V))
+        rel_x = float(torch.linalg.vector_norm(x_mixed - x) / torch.linalg.vector_norm(x))
+        max_v = float((V_mixed - V).abs().max() / V.abs().max())
+        out["fp64_gpu"] = {"value": v64, "unit": "pairs/s", "ms_per_step": w64 * 1e3,
+                           "fwd_ms": e[0].elapsed_time(e[1]), "adj_ms": e[2].elapsed_time(e[3]),
+                           "speedup_mixed_over_fp64": value / v64, "rel_l2_forward": rel_v,
+                           "rel_l2_adjoint": rel_x, "max_err_forward_rel_to_max": max_v}
+        log(f"[bench] fp64 GPU: {w64 * 1e3:.1f} ms/step; mixed/fp64 {value / v64:.2f}x; rel L2 fwd {rel_v:.2e} adj {rel_x:.2e}")
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R)
+
+    if not args.no_extras:
+        out["hbm_passes"] = hbm_passes(ops, inp, wl, dev)
+
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
+    """FP64 oracle (OpenMP on the host cores) on a bounded visibility sample."""
+    import torch
+
+    import oracle
+
+    oracle.build()
+    l, m, n = (lmn[i].cpu().numpy() for i in range(3))
+    Ih = I.cpu().numpy()
+    K = inp.uvw.shape[1]
+    rng = np.random.default_rng(123)
+
+    def run(rows):
+        u, a, b = inp.uvw[:, rows], inp.xyz_a[:, rows], inp.xyz_b[:, rows]
+        t0 = time.perf_counter()
+        Vs = oracle.forward(l, m, n, Ih, u, a, b, flags=oracle.MASK, R_moon=R)
+        t1 = time.perf_counter()
+        return Vs, t1 - t0
+
+    probe = np.sort(rng.choice(K, 16, replace=False))
+    _, tp = run(probe)
+    per_row = max(tp / probe.size, 1e-6)
+    n_rows = int(min(K, max(16, args.cpu_seconds / 2 / per_row)))
+    rows = np.sort(rng.choice(K, n_rows, replace=False))
+    log(f"[bench] cpu baseline: {n_rows} rows (probe {per_row * 1e3:.2f} ms/row)")
+    Vs, tf = run(rows)
+    # adjoint on the same rows (unit weights) with random y
+    ys = rng.normal(size=n_rows) + 1j * rng.normal(size=n_rows)
+    t0 = time.perf_counter()
+    xs = oracle.backproject(l, m, n, inp.uvw[:, rows], ys, None, inp.xyz_a[:, rows], inp.xyz_b[:, rows],
+                            flags=oracle.MASK, R_moon=R)
+    ta = time.perf_counter() - t0
+    # exact visible-pair count of the sample, from the GPU counter
+    rr = torch.from_numpy(rows).to(uvw.device)
+    vis_s = int(sky.count_visible(xa[:, rr].contiguous(), xb[:, rr].contiguous(), R_moon=R).item())
+    Vg = V_mixed[rr].cpu().numpy()
+    rel = float(np.linalg.norm(Vg - Vs) / np.linalg.norm(Vs))
+    mx = float(np.max(np.abs(Vg - Vs)) / np.max(np.abs(Vs)))
+    # mixed GPU adjoint on the same sample for the adjoint's error vs oracle
+    xg = sky.backproject(uvw[:, rr].contiguous(), torch.from_numpy(ys).to(uvw.device), None,
+                         xa[:, rr].contiguous(), xb[:, rr].contiguous(), mask=True, R_moon=R).cpu().numpy()
+    rel_x = float(np.linalg.norm(xg - xs) / np.linalg.norm(xs))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 2 * vis_s / (tf + ta), "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + adjoint "
+                      f"(OpenMP), {tf + ta:.1f} s",
+            "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x}
+
+
+def hbm_passes(ops, inp, wl, dev):
+    """HBM rates of the O(P) / O(K) passes (algorithmic bytes / event time)."""
+    import torch
+
+    res = {}
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    for nside in (256, 4096):
+        P = 12 * nside * nside
+        th = torch.empty(P, dtype=torch.float64, device=dev)
+        ph = torch.empty(P, dtype=torch.float64, device=dev)
+        from viss_optimize_3d_amd import _lib
+
+        L = _lib.load()
+
+        def f():
+            ops.check("viss_pix2ang", L.viss_pix2ang(nside, _lib.SCHEME_RING, ops._p(th), ops._p(ph), None, None,
+                                                     ops._stream()))
+
+        t = timed(f)
+        res[f"pix2ang_ring_nside{nside}"] = {"bytes": 16 * P, "s": t, "GB_per_s": 16 * P / t / 1e9,
+                                             "frac_hbm": 16 * P / t / PEAK_HBM}
+        del th, ph
+    from viss_optimize_3d_amd import geometry as geo
+
+    for rows in (2 * inp.uvw.shape[1], 40_000_000):
+        rng = np.random.default_rng(1)
+        bll = torch.from_numpy(rng.uniform(0, 1.2 * geo.REF_BL_MAX / wl.wavelength * 2, rows).astype(np.float32)).to(dev) \
+            if rows != 2 * inp.uvw.shape[1] else torch.from_numpy(np.concatenate([inp.bll, inp.bll]).astype(np.float32)).to(dev)
+        w = (torch.rand(rows, dtype=torch.float64, device=dev) * 2 - 1) * bll.double() * 0.9
+        nr = int(math.ceil(float(bll.max()) * 2))  # cover every shell (no out-of-range rows)
+        t = timed(lambda: ops.weights(bll, w, nr, want_dg=False))
+        # algorithmic bytes per row: hist pass reads bll 4 + writes gs 4; row pass reads bll 4, w 8, gs 4, writes wt 8
+        byt = 32 * rows
+        res[f"weights_rows{rows}"] = {"nr": nr, "bytes": byt, "s": t, "GB_per_s": byt / t / 1e9,
+                                      "frac_hbm": byt / t / PEAK_HBM}
+    return res
+
+
+if __name__ == "__main__":
+    main()
