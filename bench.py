@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 GPU comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the pix2ang / weights HBM measurements")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched persistent-launch comparison")
     return ap.parse_args()
 
 
@@ -186,6 +187,11 @@ def main():
     if not args.no_extras:
         out["hbm_passes"] = hbm_passes(ops, inp, wl, dev)
 
+    if not args.no_batched:
+        del sky
+        torch.cuda.empty_cache()
+        out["batched"] = batched(ops, geo, dev)
+
     print(json.dumps(out), flush=True)
 
 
@@ -236,6 +242,59 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
             "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + adjoint "
                       f"(OpenMP), {tf + ta:.1f} s",
             "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x}
+
+
+def batched(ops, geo, dev):
+    """5 frequencies x 64 snapshot groups x 500 snapshots x 28 baselines at nside 128: one persistent
+    (hipGraph-replayed, autotuned) launch against one launch per group."""
+    import torch
+
+    B, uvw, xa, xb, offs, gf = geo.batched_workload()
+    th, ph = ops.pix2ang(128, nest=False, with_ring=False)
+    lmn, I = ops.pixel_dirs(th, ph, torch.from_numpy(B).to(dev))
+    sky = ops.Sky(lmn)
+    U, A, Bx = (torch.from_numpy(a).to(dev) for a in (uvw, xa, xb))
+    R = geo.R_MOON
+    visible = int(sky.count_visible(A, Bx, R_moon=R).item())
+    G = len(offs) - 1
+    groups = [(U[:, offs[g]:offs[g + 1]].contiguous(), A[:, offs[g]:offs[g + 1]].contiguous(),
+               Bx[:, offs[g]:offs[g + 1]].contiguous()) for g in range(G)]
+    V = torch.zeros(U.shape[1], dtype=torch.complex128, device=dev)
+    log(f"[bench] batched: {G} groups, {U.shape[1]} rows, visible pairs {visible}")
+
+    def per_group():
+        for g, (u, a, b) in enumerate(groups):
+            sky.forward(I, u, a, b, mask=True, R_moon=R, out=V[offs[g]:offs[g + 1]])
+
+    batch = ops.Batch(sky, offs, U, A, Bx, mask=True, R_moon=R)
+    st0 = batch.stats()
+    side = torch.cuda.Stream()
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    t_group = timed(per_group)
+    t_default = timed(lambda: batch.forward(I, out=V))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.autotune(I, reps=1)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
+    with torch.cuda.stream(side):
+        t_graph = timed(lambda: batch.forward(I, out=V, use_graph=True))
+    torch.cuda.synchronize()
+    st = batch.stats()
+    return {"groups": G, "rows": int(U.shape[1]), "npix": sky.P, "visible_pairs": visible,
+            "visible_tile_fraction": st["visible_tiles"] / st["total_tiles"],
+            "per_group_launches_pairs_per_s": visible / t_group, "batched_default_pairs_per_s": visible / t_default,
+            "batched_tuned_graph_pairs_per_s": visible / t_graph, "speedup_batched_over_per_group": t_group / t_graph,
+            "autotune_s": t_tune, "work_items": st["n_items"], "work_items_untuned": st0["n_items"]}
 
 
 def hbm_passes(ops, inp, wl, dev):

@@ -139,6 +139,35 @@ int viss_sky_backproject(viss_sky* sky, const double* uvw, const double* xyz_a, 
 int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,
                            const viss_mask_geometry* geom, int64_t* count, void* stream);
 
+/* ---- batched forward: many visibility groups (snapshot groups x
+ * frequencies) over one sky in ONE persistent launch.  Replaces the
+ * reference's per-day loop of forward launches (vissgen3d_noblockage.cu:
+ * 270-440) for in-memory batches.  Rows of group g are rows
+ * [row_offsets[g], row_offsets[g+1]) of the SoA 3 x K_total arrays uvw /
+ * xyz_a / xyz_b (device; they must stay valid for the batch's lifetime);
+ * row_offsets is a HOST array of n_groups + 1 entries starting at 0.  Each
+ * group gets a compacted list of the sky tiles its satellites can see
+ * (uneven lists are balanced by an atomic work queue).  flags: VISS_MASK,
+ * VISS_PHASE_TRACK, VISS_PREC_FP64.  The batch keeps a pointer to `sky`. */
+typedef struct viss_batch viss_batch;
+
+int viss_batch_create(viss_sky* sky, int64_t n_groups, const int64_t* row_offsets, const double* uvw,
+                      const double* xyz_a, const double* xyz_b, const viss_mask_geometry* geom, int flags,
+                      viss_batch** batch, void* stream);
+int viss_batch_destroy(viss_batch* batch);
+/* V[k] += forward for every row k of every group (complex [K_total]).
+ * use_graph != 0: the brightness scatter, the queue reset and the persistent
+ * kernel are captured once into a hipGraph (re-captured if I, V or the stream
+ * change) and replayed; needs a non-default stream. */
+int viss_batch_forward(viss_batch* batch, const double* I, double* V, int use_graph, void* stream);
+/* Times candidate tile-range lengths per (log2 visible tiles, log2 rows)
+ * bucket on the device (reps launches each, accumulating into V_scratch
+ * [K_total] complex) and keeps the fastest; synchronous. */
+int viss_batch_autotune(viss_batch* batch, const double* I, double* V_scratch, int reps, void* stream);
+int viss_batch_tuning_save(const viss_batch* batch, const char* path);
+int viss_batch_tuning_load(viss_batch* batch, const char* path, void* stream);
+int viss_batch_stats(const viss_batch* batch, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items);
+
 #ifdef __cplusplus
 }
 #endif

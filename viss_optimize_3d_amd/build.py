@@ -24,7 +24,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE, "-I", CSRC,
           "-Wall", "-Wno-unused-function"] + DEVICE_FLAGS
 
-SOURCES = ["viss_sky.hip", "viss_pix2ang.hip", "viss_nudft.hip", "viss_weights.hip"]
+SOURCES = ["viss_sky.hip", "viss_pix2ang.hip", "viss_nudft.hip", "viss_weights.hip", "viss_batch.hip"]
 HOST_PROGRAMS = {"pix2ang": "host/pix2ang_main.cpp", "vissgen3d": "host/vissgen3d_main.cpp"}
 
 
@@ -44,7 +44,7 @@ def _newer(target, deps):
 
 def build(force: bool = True, verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    headers = [os.path.join(CSRC, "viss_internal.h"), os.path.join(INCLUDE, "viss.h")]
+    headers = [os.path.join(CSRC, "viss_internal.h"), os.path.join(CSRC, "viss_kernels.h"), os.path.join(INCLUDE, "viss.h")]
     jobs = []
     objs = []
     for src in SOURCES:

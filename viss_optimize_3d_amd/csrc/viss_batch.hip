@@ -1,0 +1,527 @@
+// Batched forward over many visibility groups (snapshot groups x frequencies)
+// that share one sky, in ONE persistent launch, replayable as a hipGraph.
+//
+// The reference runs one day per loop iteration, one launch per kernel, with a
+// device sync after each (vissgen3d_noblockage.cu:270-557).  With occultation
+// each group only sees part of the sky, unevenly: here every group gets a
+// compacted list of the 64-pixel tiles that any of its satellites can see
+// (a conservative cone test of the tile against the bounding cone of the
+// group's satellite directions), and a work table of (group, 256 rows, range
+// of its tile list) items is drained by a persistent grid through an atomic
+// counter.  The per-item tile-range length is chosen per
+// (visible-tile-count, row-count) bucket by an autotuner that times the
+// candidates on the device (viss_batch_autotune), and the table can be saved
+// and reloaded.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "viss_kernels.h"
+
+namespace viss {
+
+struct WorkItem {
+  int64_t row0;
+  int32_t g, nrows, j0, j1;
+  int32_t pad0, pad1;
+};
+static_assert(sizeof(WorkItem) == 32, "WorkItem is 32 B");
+
+// Bounding cone of the satellite directions of each group and the lowest unit
+// cut among them.  One block per group.
+__global__ __launch_bounds__(256) void k_group_bounds(const double* __restrict__ xyza, const double* __restrict__ xyzb,
+                                                      int64_t Ktot, const int64_t* __restrict__ offs, double R_moon,
+                                                      double orbit_r, float4* __restrict__ gcone,
+                                                      float* __restrict__ gcut) {
+  __shared__ double red[4][4];
+  __shared__ double dir[4];
+  const int g = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t o0 = offs[g], o1 = offs[g + 1];
+  double sx = 0, sy = 0, sz = 0, cmin = 2.0;
+  for (int64_t k = o0 + threadIdx.x; k < o1; k += 256) {
+    VisMask m;
+    VisCls c;
+    make_mask_records(xyza[k], xyza[Ktot + k], xyza[2 * Ktot + k], xyzb[k], xyzb[Ktot + k], xyzb[2 * Ktot + k],
+                      R_moon, orbit_r, m, c);
+    sx += (double)c.ax + (double)c.bx;
+    sy += (double)c.ay + (double)c.by;
+    sz += (double)c.az + (double)c.bz;
+    cmin = fmin(cmin, fmin((double)c.ca, (double)c.cb));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sx += __shfl_xor(sx, o, 64);
+    sy += __shfl_xor(sy, o, 64);
+    sz += __shfl_xor(sz, o, 64);
+    cmin = fmin(cmin, __shfl_xor(cmin, o, 64));
+  }
+  if (lane == 0) {
+    red[wv][0] = sx;
+    red[wv][1] = sy;
+    red[wv][2] = sz;
+    red[wv][3] = cmin;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double X = 0, Y = 0, Z = 0, C = 2.0;
+    for (int i = 0; i < 4; ++i) {
+      X += red[i][0];
+      Y += red[i][1];
+      Z += red[i][2];
+      C = fmin(C, red[i][3]);
+    }
+    const double nrm = sqrt(X * X + Y * Y + Z * Z);
+    dir[0] = nrm > 1e-9 ? X / nrm : 0.0;
+    dir[1] = nrm > 1e-9 ? Y / nrm : 0.0;
+    dir[2] = nrm > 1e-9 ? Z / nrm : 1.0;
+    dir[3] = nrm > 1e-9 ? C : -3.0;  // degenerate group: everything possibly visible
+  }
+  __syncthreads();
+  double mx = 0.0;
+  for (int64_t k = o0 + threadIdx.x; k < o1; k += 256) {
+    VisMask m;
+    VisCls c;
+    make_mask_records(xyza[k], xyza[Ktot + k], xyza[2 * Ktot + k], xyzb[k], xyzb[Ktot + k], xyzb[2 * Ktot + k],
+                      R_moon, orbit_r, m, c);
+    const double ax = c.ax - dir[0], ay = c.ay - dir[1], az = c.az - dir[2];
+    const double bx = c.bx - dir[0], by = c.by - dir[1], bz = c.bz - dir[2];
+    mx = fmax(mx, fmax(sqrt(ax * ax + ay * ay + az * az), sqrt(bx * bx + by * by + bz * bz)));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  __syncthreads();
+  if (lane == 0) red[wv][0] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double M = 0;
+    for (int i = 0; i < 4; ++i) M = fmax(M, red[i][0]);
+    gcone[g] = make_float4((float)dir[0], (float)dir[1], (float)dir[2], (float)(M * (1.0 + 1e-6) + 1e-6));
+    gcut[g] = (float)dir[3];
+  }
+}
+
+// Compacted list of tiles some satellite of the group may see.  One block per group.
+__global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ cone, int64_t ntile,
+                                                     const float4* __restrict__ gcone, const float* __restrict__ gcut,
+                                                     int32_t* __restrict__ tiles, int32_t* __restrict__ count) {
+  __shared__ int wcount[4];
+  const int g = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const float4 gc = gcone[g];
+  const float cut = gcut[g];
+  int32_t* out = tiles + (int64_t)g * ntile;
+  int running = 0;
+  for (int64_t base = 0; base < ntile; base += 256) {
+    const int64_t t = base + threadIdx.x;
+    bool keep = false;
+    if (t < ntile) {
+      const float4 c = cone[t];
+      const float d = fmaf(c.x, gc.x, fmaf(c.y, gc.y, c.z * gc.z));
+      keep = d + c.w + gc.w >= cut - kClsEps;
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) wcount[wv] = __popcll(bal);
+    __syncthreads();
+    int before = running;
+    for (int i = 0; i < wv; ++i) before += wcount[i];
+    const int within = __popcll(bal & ((1ull << lane) - 1ull));
+    if (keep) out[before + within] = (int32_t)t;
+    const int total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+    running += total;
+  }
+  if (threadIdx.x == 0) count[g] = running;
+}
+
+__global__ void k_reset_counter(unsigned int* c) { *c = 0u; }
+
+template <bool MASK, bool F64>
+__global__ __launch_bounds__(256) void k_forward_batch(const PixPack* __restrict__ pix,
+                                                       const double* __restrict__ I64,
+                                                       const float4* __restrict__ cone,
+                                                       const int32_t* __restrict__ tiles, int64_t ntile,
+                                                       const WorkItem* __restrict__ items, int64_t n_items,
+                                                       unsigned int* __restrict__ counter,
+                                                       const double* __restrict__ uvw,
+                                                       const double* __restrict__ xyza,
+                                                       const double* __restrict__ xyzb, int64_t Ktot, double R_moon,
+                                                       double orbit_r, int track, double* __restrict__ V) {
+  __shared__ int64_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int64_t it = s_item;
+    __syncthreads();
+    if (it >= n_items) break;  // block-uniform: every wave leaves once the table is drained
+    const WorkItem w = items[it];
+    const int64_t k = w.row0 + threadIdx.x;
+    const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, (int)threadIdx.x < w.nrows, R_moon, orbit_r, track);
+    double Sr = 0.0, Si = 0.0;
+    forward_tiles<MASK, F64>(pix, I64, cone, tiles + (int64_t)w.g * ntile, w.j0, w.j1, r, Sr, Si);
+    if (r.act) forward_store(V, k, Ktot, 0, Sr, Si);
+  }
+}
+
+}  // namespace viss
+
+using namespace viss;
+
+struct viss_batch {
+  viss_sky* sky = nullptr;
+  int device = -1;
+  int64_t G = 0, Ktot = 0;
+  std::vector<int64_t> offs;
+  const double *uvw = nullptr, *xa = nullptr, *xb = nullptr;
+  double R = 0, orb = 0;
+  int flags = 0;
+  int64_t* d_offs = nullptr;
+  float4* gcone = nullptr;
+  float* gcut = nullptr;
+  int32_t* tiles = nullptr;
+  int32_t* d_count = nullptr;
+  std::vector<int32_t> ntiles_g;
+  std::map<int, int> tune;  // bucket -> tiles per item
+  WorkItem* items = nullptr;
+  int64_t n_items = 0;
+  unsigned int* counter = nullptr;
+  int grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const double* g_I = nullptr;
+  double* g_V = nullptr;
+  hipStream_t g_stream = nullptr;
+};
+
+namespace viss {
+
+static int bucket_of(int64_t nt, int64_t rows) {
+  int a = 0, b = 0;
+  while ((1ll << (a + 1)) <= nt) ++a;
+  while ((1ll << (b + 1)) <= rows) ++b;
+  return a * 64 + b;
+}
+
+static int default_tpc(int64_t nt) {
+  int64_t t = nt / 8;
+  if (t < 16) t = 16;
+  if (t > 256) t = 256;
+  return (int)t;
+}
+
+static void build_items(const viss_batch* b, const std::vector<int>& groups, std::vector<WorkItem>& out,
+                        const std::map<int, int>& tune, int force_tpc) {
+  out.clear();
+  for (int g : groups) {
+    const int64_t rows = b->offs[g + 1] - b->offs[g];
+    const int64_t nt = b->ntiles_g[g];
+    if (rows <= 0 || nt <= 0) continue;
+    int tpc = force_tpc;
+    if (tpc <= 0) {
+      auto it = tune.find(bucket_of(nt, rows));
+      tpc = it != tune.end() ? it->second : default_tpc(nt);
+    }
+    for (int64_t r0 = 0; r0 < rows; r0 += 256)
+      for (int64_t j = 0; j < nt; j += tpc) {
+        WorkItem w;
+        w.row0 = b->offs[g] + r0;
+        w.g = g;
+        w.nrows = (int32_t)std::min<int64_t>(256, rows - r0);
+        w.j0 = (int32_t)j;
+        w.j1 = (int32_t)std::min<int64_t>(nt, j + tpc);
+        w.pad0 = w.pad1 = 0;
+        out.push_back(w);
+      }
+  }
+  // largest items first: the atomic queue then ends on small ones
+  std::stable_sort(out.begin(), out.end(), [](const WorkItem& a, const WorkItem& c) {
+    return (int64_t)a.nrows * (a.j1 - a.j0) > (int64_t)c.nrows * (c.j1 - c.j0);
+  });
+}
+
+static int upload_items(viss_batch* b, const std::vector<WorkItem>& v, hipStream_t s) {
+  if (b->items) VISS_HIP(hipFree(b->items));
+  b->items = nullptr;
+  b->n_items = (int64_t)v.size();
+  if (!v.empty()) {
+    VISS_HIP(hipMalloc((void**)&b->items, v.size() * sizeof(WorkItem)));
+    VISS_HIP(hipMemcpyAsync(b->items, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    VISS_HIP(hipStreamSynchronize(s));
+  }
+  // invalidate a captured graph (it holds the old table)
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  return VISS_OK;
+}
+
+static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, double* V, hipStream_t s) {
+  viss_sky* sky = b->sky;
+  k_reset_counter<<<1, 1, 0, s>>>(b->counter);
+  VISS_LAUNCHED(k_reset_counter);
+  if (n_items == 0) return VISS_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>(b->grid, n_items);
+  const int track = (b->flags & VISS_PHASE_TRACK) ? 1 : 0;
+  const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
+#define VISS_FB(M, F)                                                                                            \
+  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->I64, sky->cone, b->tiles, sky->ntile, items, n_items, \
+                                             b->counter, b->uvw, b->xa, b->xb, b->Ktot, b->R, b->orb, track, V)
+  if (mask) {
+    if (f64) VISS_FB(true, true); else VISS_FB(true, false);
+  } else {
+    if (f64) VISS_FB(false, true); else VISS_FB(false, false);
+  }
+#undef VISS_FB
+  VISS_LAUNCHED(k_forward_batch);
+  return VISS_OK;
+}
+
+static void batch_free(viss_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_offs) (void)hipFree(b->d_offs);
+  if (b->gcone) (void)hipFree(b->gcone);
+  if (b->gcut) (void)hipFree(b->gcut);
+  if (b->tiles) (void)hipFree(b->tiles);
+  if (b->d_count) (void)hipFree(b->d_count);
+  if (b->items) (void)hipFree(b->items);
+  if (b->counter) (void)hipFree(b->counter);
+}
+
+static int batch_build(viss_batch* b, hipStream_t s) {
+  viss_sky* sky = b->sky;
+  const int64_t G = b->G;
+  VISS_HIP(hipMalloc((void**)&b->d_offs, sizeof(int64_t) * (G + 1)));
+  VISS_HIP(hipMalloc((void**)&b->gcone, sizeof(float4) * G));
+  VISS_HIP(hipMalloc((void**)&b->gcut, sizeof(float) * G));
+  VISS_HIP(hipMalloc((void**)&b->tiles, sizeof(int32_t) * G * sky->ntile));
+  VISS_HIP(hipMalloc((void**)&b->d_count, sizeof(int32_t) * G));
+  VISS_HIP(hipMalloc((void**)&b->counter, sizeof(unsigned int)));
+  VISS_HIP(hipMemcpyAsync(b->d_offs, b->offs.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, s));
+  b->ntiles_g.assign(G, (int32_t)sky->ntile);
+  std::vector<int32_t> all;
+  if (b->flags & VISS_MASK) {
+    k_group_bounds<<<(unsigned)G, 256, 0, s>>>(b->xa, b->xb, b->Ktot, b->d_offs, b->R, b->orb, b->gcone, b->gcut);
+    VISS_LAUNCHED(k_group_bounds);
+    k_group_tiles<<<(unsigned)G, 256, 0, s>>>(sky->cone, sky->ntile, b->gcone, b->gcut, b->tiles, b->d_count);
+    VISS_LAUNCHED(k_group_tiles);
+    VISS_HIP(hipMemcpyAsync(b->ntiles_g.data(), b->d_count, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
+  } else {
+    all.resize(sky->ntile);
+    for (int64_t t = 0; t < sky->ntile; ++t) all[t] = (int32_t)t;
+    for (int64_t g = 0; g < G; ++g)
+      VISS_HIP(hipMemcpyAsync(b->tiles + g * sky->ntile, all.data(), sizeof(int32_t) * sky->ntile,
+                              hipMemcpyHostToDevice, s));
+  }
+  VISS_HIP(hipStreamSynchronize(s));
+  int dev = 0, ncu = 0, per_cu = 0;
+  VISS_HIP(hipGetDevice(&dev));
+  VISS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
+  const void* kern = mask ? (f64 ? (const void*)k_forward_batch<true, true> : (const void*)k_forward_batch<true, false>)
+                          : (f64 ? (const void*)k_forward_batch<false, true> : (const void*)k_forward_batch<false, false>);
+  VISS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+  b->grid = std::max(1, ncu * std::max(1, per_cu));
+  std::vector<int> groups(G);
+  for (int64_t g = 0; g < G; ++g) groups[g] = (int)g;
+  std::vector<WorkItem> v;
+  build_items(b, groups, v, b->tune, 0);
+  return upload_items(b, v, s);
+}
+
+}  // namespace viss
+
+extern "C" {
+
+int viss_batch_create(viss_sky* sky, int64_t n_groups, const int64_t* row_offsets, const double* uvw,
+                      const double* xyz_a, const double* xyz_b, const viss_mask_geometry* geom, int flags,
+                      viss_batch** out, void* stream) {
+  VISS_REQUIRE(out != nullptr, "viss_batch_create: out is NULL");
+  *out = nullptr;
+  VISS_REQUIRE(sky != nullptr && row_offsets != nullptr && uvw != nullptr, "viss_batch_create: NULL argument");
+  VISS_REQUIRE(n_groups >= 1 && n_groups <= (1 << 20), "viss_batch_create: n_groups %lld out of range",
+               (long long)n_groups);
+  VISS_REQUIRE((flags & ~(VISS_MASK | VISS_PHASE_TRACK | VISS_PREC_FP64)) == 0,
+               "viss_batch_create: unsupported flags 0x%x", flags);
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE(row_offsets[0] == 0, "viss_batch_create: row_offsets[0] must be 0");
+  for (int64_t g = 0; g < n_groups; ++g)
+    VISS_REQUIRE(row_offsets[g + 1] >= row_offsets[g], "viss_batch_create: row_offsets must be non-decreasing");
+  const int64_t Ktot = row_offsets[n_groups];
+  VISS_REQUIRE(Ktot <= ((int64_t)1 << 40), "viss_batch_create: too many rows");
+  if (flags & VISS_MASK) {
+    VISS_REQUIRE(geom && xyz_a && xyz_b, "viss_batch_create: VISS_MASK needs geometry and satellite positions");
+    VISS_REQUIRE(geom->R_moon >= 0.0 && (geom->orbit_radius <= 0.0 || geom->orbit_radius > geom->R_moon),
+                 "viss_batch_create: bad geometry");
+  }
+  viss_batch* b = new (std::nothrow) viss_batch();
+  if (!b) {
+    set_error("viss_batch_create: host allocation failed");
+    return VISS_ERR_NOMEM;
+  }
+  b->sky = sky;
+  b->device = sky->device;
+  b->G = n_groups;
+  b->Ktot = Ktot;
+  b->offs.assign(row_offsets, row_offsets + n_groups + 1);
+  b->uvw = uvw;
+  b->xa = xyz_a;
+  b->xb = xyz_b;
+  b->R = (flags & VISS_MASK) ? geom->R_moon : 0.0;
+  b->orb = (flags & VISS_MASK) ? geom->orbit_radius : 0.0;
+  b->flags = flags;
+  rc = batch_build(b, as_stream(stream));
+  if (rc != VISS_OK) {
+    batch_free(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return VISS_OK;
+}
+
+int viss_batch_destroy(viss_batch* b) {
+  if (!b) return VISS_OK;
+  batch_free(b);
+  delete b;
+  return VISS_OK;
+}
+
+int viss_batch_forward(viss_batch* b, const double* I, double* V, int use_graph, void* stream) {
+  VISS_REQUIRE(b != nullptr && I != nullptr && V != nullptr, "viss_batch_forward: NULL argument");
+  int rc = check_device(b->sky);
+  if (rc != VISS_OK) return rc;
+  hipStream_t s = as_stream(stream);
+  VISS_REQUIRE(!use_graph || s != nullptr,
+               "viss_batch_forward: graph mode needs a non-default stream (capture is not allowed on stream 0)");
+  if (!use_graph) {
+    rc = sky_set_brightness(b->sky, I, (b->flags & VISS_PREC_FP64) != 0, s);
+    if (rc != VISS_OK) return rc;
+    return launch_batch(b, b->items, b->n_items, V, s);
+  }
+  if (!b->exec || b->g_I != I || b->g_V != V || b->g_stream != s) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    VISS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = sky_set_brightness(b->sky, I, (b->flags & VISS_PREC_FP64) != 0, s);
+    if (rc == VISS_OK) rc = launch_batch(b, b->items, b->n_items, V, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != VISS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    b->graph = g;
+    VISS_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->g_I = I;
+    b->g_V = V;
+    b->g_stream = s;
+  }
+  VISS_HIP(hipGraphLaunch(b->exec, s));
+  return VISS_OK;
+}
+
+int viss_batch_autotune(viss_batch* b, const double* I, double* V_scratch, int reps, void* stream) {
+  VISS_REQUIRE(b != nullptr && I != nullptr && V_scratch != nullptr, "viss_batch_autotune: NULL argument");
+  VISS_REQUIRE(reps >= 1 && reps <= 100, "viss_batch_autotune: reps %d outside [1, 100]", reps);
+  int rc = check_device(b->sky);
+  if (rc != VISS_OK) return rc;
+  hipStream_t s = as_stream(stream);
+  rc = sky_set_brightness(b->sky, I, (b->flags & VISS_PREC_FP64) != 0, s);
+  if (rc != VISS_OK) return rc;
+  std::map<int, std::vector<int>> buckets;
+  for (int64_t g = 0; g < b->G; ++g) {
+    const int64_t rows = b->offs[g + 1] - b->offs[g];
+    if (rows > 0 && b->ntiles_g[g] > 0) buckets[bucket_of(b->ntiles_g[g], rows)].push_back((int)g);
+  }
+  static const int cands[] = {8, 16, 32, 64, 128, 256, 1 << 20};
+  hipEvent_t e0, e1;
+  VISS_HIP(hipEventCreate(&e0));
+  VISS_HIP(hipEventCreate(&e1));
+  WorkItem* tmp = nullptr;
+  size_t tmp_n = 0;
+  for (auto& kv : buckets) {
+    float best = 1e30f;
+    int best_tpc = default_tpc(b->ntiles_g[kv.second[0]]);
+    for (int tpc : cands) {
+      std::vector<WorkItem> v;
+      build_items(b, kv.second, v, b->tune, tpc);
+      if (v.size() > tmp_n) {
+        if (tmp) VISS_HIP(hipFree(tmp));
+        VISS_HIP(hipMalloc((void**)&tmp, v.size() * sizeof(WorkItem)));
+        tmp_n = v.size();
+      }
+      VISS_HIP(hipMemcpyAsync(tmp, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+      rc = launch_batch(b, tmp, (int64_t)v.size(), V_scratch, s);  // warm
+      if (rc != VISS_OK) break;
+      VISS_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps && rc == VISS_OK; ++r) rc = launch_batch(b, tmp, (int64_t)v.size(), V_scratch, s);
+      VISS_HIP(hipEventRecord(e1, s));
+      VISS_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      VISS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_tpc = tpc;
+      }
+    }
+    if (rc != VISS_OK) break;
+    b->tune[kv.first] = best_tpc;
+  }
+  if (tmp) (void)hipFree(tmp);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != VISS_OK) return rc;
+  std::vector<int> groups(b->G);
+  for (int64_t g = 0; g < b->G; ++g) groups[g] = (int)g;
+  std::vector<WorkItem> v;
+  build_items(b, groups, v, b->tune, 0);
+  return upload_items(b, v, s);
+}
+
+int viss_batch_tuning_save(const viss_batch* b, const char* path) {
+  VISS_REQUIRE(b != nullptr && path != nullptr, "viss_batch_tuning_save: NULL argument");
+  FILE* f = fopen(path, "w");
+  VISS_REQUIRE(f != nullptr, "viss_batch_tuning_save: cannot open %s", path);
+  fprintf(f, "# viss batch tuning: log2(visible tiles) log2(rows) tiles_per_item\n");
+  for (auto& kv : b->tune) fprintf(f, "%d %d %d\n", kv.first / 64, kv.first % 64, kv.second);
+  fclose(f);
+  return VISS_OK;
+}
+
+int viss_batch_tuning_load(viss_batch* b, const char* path, void* stream) {
+  VISS_REQUIRE(b != nullptr && path != nullptr, "viss_batch_tuning_load: NULL argument");
+  FILE* f = fopen(path, "r");
+  VISS_REQUIRE(f != nullptr, "viss_batch_tuning_load: cannot open %s", path);
+  char line[256];
+  std::map<int, int> t;
+  while (fgets(line, sizeof line, f)) {
+    int a, c, tpc;
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%d %d %d", &a, &c, &tpc) == 3 && a >= 0 && a < 64 && c >= 0 && c < 64 && tpc >= 1)
+      t[a * 64 + c] = tpc;
+  }
+  fclose(f);
+  b->tune = t;
+  std::vector<int> groups(b->G);
+  for (int64_t g = 0; g < b->G; ++g) groups[g] = (int)g;
+  std::vector<WorkItem> v;
+  build_items(b, groups, v, b->tune, 0);
+  return upload_items(b, v, as_stream(stream));
+}
+
+int viss_batch_stats(const viss_batch* b, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items) {
+  VISS_REQUIRE(b != nullptr, "viss_batch_stats: NULL batch");
+  int64_t vt = 0;
+  for (int32_t c : b->ntiles_g) vt += c;
+  if (visible_tiles) *visible_tiles = vt;
+  if (total_tiles) *total_tiles = b->G * b->sky->ntile;
+  if (n_items) *n_items = b->n_items;
+  return VISS_OK;
+}
+
+}  // extern "C"

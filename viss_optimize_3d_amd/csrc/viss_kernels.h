@@ -1,0 +1,154 @@
+// Device-side building blocks shared by the forward / back-projection kernels
+// (viss_nudft.hip) and the batched persistent forward (viss_batch.hip).
+#pragma once
+
+#include "viss_internal.h"
+
+namespace viss {
+
+// ------------------------------------------------------------ pair math
+__device__ __forceinline__ float rev_fp32(double ph) {
+  return (float)__builtin_amdgcn_fract(ph);  // [0,1) revolutions, exact in FP64
+}
+
+__device__ __forceinline__ void sincos_rev(float r, float& s, float& c) {
+  s = __builtin_amdgcn_sinf(r);  // sin(2 pi r)
+  c = __builtin_amdgcn_cosf(r);
+}
+
+__device__ __forceinline__ bool both_visible(double l, double m, double n, const VisMask& q) {
+  const double da = fma(l, q.xa, fma(m, q.ya, n * q.za));
+  const double db = fma(l, q.xb, fma(m, q.yb, n * q.zb));
+  return (da >= q.ta) & (db >= q.tb);
+}
+
+// Conservative cone test of one visibility's two satellites against a tile.
+__device__ __forceinline__ void classify(const float4 cn, const VisCls& c, bool& any_v, bool& all_v) {
+  const float da = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+  const float db = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, cn.z * c.bz));
+  any_v = (da + cn.w >= c.ca - kClsEps) & (db + cn.w >= c.cb - kClsEps);
+  all_v = (da - cn.w >= c.ca + kClsEps) & (db - cn.w >= c.cb + kClsEps);
+}
+
+__device__ __forceinline__ void make_mask_records(double xa, double ya, double za, double xb, double yb, double zb,
+                                                  double R_moon, double orbit_r, VisMask& m, VisCls& c) {
+  m.xa = xa; m.ya = ya; m.za = za;
+  m.xb = xb; m.yb = yb; m.zb = zb;
+  m.ta = mask_threshold(xa, ya, za, R_moon, orbit_r);
+  m.tb = mask_threshold(xb, yb, zb, R_moon, orbit_r);
+  const double ra = sqrt(xa * xa + ya * ya + za * za), rb = sqrt(xb * xb + yb * yb + zb * zb);
+  const double ia = ra > 0 ? 1.0 / ra : 0.0, ib = rb > 0 ? 1.0 / rb : 0.0;
+  c.ax = (float)(xa * ia); c.ay = (float)(ya * ia); c.az = (float)(za * ia);
+  c.bx = (float)(xb * ib); c.by = (float)(yb * ib); c.bz = (float)(zb * ib);
+  // a satellite at the origin sees everything with t = 0 only if s.0 >= 0: always true
+  c.ca = ra > 0 ? (float)(m.ta * ia) : -2.0f;
+  c.cb = rb > 0 ? (float)(m.tb * ib) : -2.0f;
+}
+
+// ------------------------------------------------------------ forward tile loop
+// One visibility row per lane: phase coefficients and occultation records.
+struct FwdRow {
+  double u, v, w, c0;
+  VisMask mk;
+  VisCls cl;
+  bool act;
+};
+
+template <bool MASK>
+__device__ __forceinline__ FwdRow load_fwd_row(const double* __restrict__ uvw, const double* __restrict__ xyza,
+                                               const double* __restrict__ xyzb, int64_t K, int64_t k, bool act,
+                                               double R_moon, double orbit_r, int track) {
+  FwdRow r;
+  const int64_t kk = act ? k : K - 1;
+  r.act = act;
+  r.u = uvw[kk];
+  r.v = uvw[K + kk];
+  r.w = uvw[2 * K + kk];
+  r.c0 = track ? -r.w : 0.0;
+  r.mk = VisMask{};
+  r.cl = VisCls{};
+  if (MASK)
+    make_mask_records(xyza[kk], xyza[K + kk], xyza[2 * K + kk], xyzb[kk], xyzb[K + kk], xyzb[2 * K + kk], R_moon,
+                      orbit_r, r.mk, r.cl);
+  return r;
+}
+
+// Accumulates sum_p I_p (cos, sin)(2 pi phase) over the tiles t = list[j]
+// (t = j when list is NULL), j in [j0, j1), into (Sr, Si).  All lanes of a
+// wavefront walk the same tiles; tile data are wave-uniform.
+template <bool MASK, bool F64>
+__device__ __forceinline__ void forward_tiles(const PixPack* __restrict__ pix, const double* __restrict__ I64,
+                                              const float4* __restrict__ cone, const int32_t* __restrict__ list,
+                                              int64_t j0, int64_t j1, const FwdRow& r, double& Sr, double& Si) {
+  const double u = r.u, v = r.v, w = r.w, c0 = r.c0;
+  for (int64_t jt = j0; jt < j1; ++jt) {
+    const int64_t t = list ? (int64_t)list[jt] : jt;
+    bool full = true;
+    if (MASK) {
+      bool any_v, all_v;
+      classify(cone[t], r.cl, any_v, all_v);
+      any_v = any_v & r.act;
+      all_v = all_v | !r.act;
+      if (__ballot(any_v) == 0) continue;
+      full = __ballot(!all_v) == 0;
+    }
+    const PixPack* __restrict__ tp = pix + t * kTile;
+    if (!F64) {
+      float ar0 = 0.f, ai0 = 0.f, ar1 = 0.f, ai1 = 0.f;
+      if (full) {
+#pragma unroll 4
+        for (int j = 0; j < kTile; j += 2) {
+          const PixPack p0 = tp[j], p1 = tp[j + 1];
+          float s0, c0f, s1, c1f;
+          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
+          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
+          ar0 = fmaf(p0.I, c0f, ar0);
+          ai0 = fmaf(p0.I, s0, ai0);
+          ar1 = fmaf(p1.I, c1f, ar1);
+          ai1 = fmaf(p1.I, s1, ai1);
+        }
+      } else {
+#pragma unroll 2
+        for (int j = 0; j < kTile; j += 2) {
+          const PixPack p0 = tp[j], p1 = tp[j + 1];
+          const float I0 = both_visible(p0.l, p0.m, p0.n, r.mk) ? p0.I : 0.f;
+          const float I1 = both_visible(p1.l, p1.m, p1.n, r.mk) ? p1.I : 0.f;
+          float s0, c0f, s1, c1f;
+          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
+          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
+          ar0 = fmaf(I0, c0f, ar0);
+          ai0 = fmaf(I0, s0, ai0);
+          ar1 = fmaf(I1, c1f, ar1);
+          ai1 = fmaf(I1, s1, ai1);
+        }
+      }
+      Sr += (double)ar0 + (double)ar1;
+      Si += (double)ai0 + (double)ai1;
+    } else {
+      const double* __restrict__ ip = I64 + t * kTile;
+      for (int j = 0; j < kTile; ++j) {
+        const PixPack p0 = tp[j];
+        double I0 = ip[j];
+        if (!full && !both_visible(p0.l, p0.m, p0.n, r.mk)) I0 = 0.0;
+        const double f = __builtin_amdgcn_fract(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0))));
+        double s, c;
+        sincospi(2.0 * f, &s, &c);
+        Sr = fma(I0, c, Sr);
+        Si = fma(I0, s, Si);
+      }
+    }
+  }
+}
+
+// V_k += Sr - i Si  (V = sum I exp(-2 pi i ph)); with herm, V_{k+K} += conj.
+__device__ __forceinline__ void forward_store(double* __restrict__ V, int64_t k, int64_t K, int herm, double Sr,
+                                              double Si) {
+  unsafeAtomicAdd(&V[2 * k], Sr);
+  unsafeAtomicAdd(&V[2 * k + 1], -Si);
+  if (herm) {
+    unsafeAtomicAdd(&V[2 * (k + K)], Sr);
+    unsafeAtomicAdd(&V[2 * (k + K) + 1], Si);
+  }
+}
+
+}  // namespace viss

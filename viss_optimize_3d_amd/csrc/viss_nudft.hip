@@ -30,48 +30,9 @@
 //    pixel / visibility chunks meet through FP64 atomics in the caller's
 //    accumulate-into buffer.
 //  * VISS_PREC_FP64: same structure, FP64 sincospi and FP64 accumulation.
-#include "viss_internal.h"
+#include "viss_kernels.h"
 
 namespace viss {
-
-// ------------------------------------------------------------ pair math
-__device__ __forceinline__ float rev_fp32(double ph) {
-  return (float)__builtin_amdgcn_fract(ph);  // [0,1) revolutions, exact in FP64
-}
-
-__device__ __forceinline__ void sincos_rev(float r, float& s, float& c) {
-  s = __builtin_amdgcn_sinf(r);  // sin(2 pi r)
-  c = __builtin_amdgcn_cosf(r);
-}
-
-__device__ __forceinline__ bool both_visible(double l, double m, double n, const VisMask& q) {
-  const double da = fma(l, q.xa, fma(m, q.ya, n * q.za));
-  const double db = fma(l, q.xb, fma(m, q.yb, n * q.zb));
-  return (da >= q.ta) & (db >= q.tb);
-}
-
-// Conservative cone test of one visibility's two satellites against a tile.
-__device__ __forceinline__ void classify(const float4 cn, const VisCls& c, bool& any_v, bool& all_v) {
-  const float da = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
-  const float db = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, cn.z * c.bz));
-  any_v = (da + cn.w >= c.ca - kClsEps) & (db + cn.w >= c.cb - kClsEps);
-  all_v = (da - cn.w >= c.ca + kClsEps) & (db - cn.w >= c.cb + kClsEps);
-}
-
-__device__ __forceinline__ void make_mask_records(double xa, double ya, double za, double xb, double yb, double zb,
-                                                  double R_moon, double orbit_r, VisMask& m, VisCls& c) {
-  m.xa = xa; m.ya = ya; m.za = za;
-  m.xb = xb; m.yb = yb; m.zb = zb;
-  m.ta = mask_threshold(xa, ya, za, R_moon, orbit_r);
-  m.tb = mask_threshold(xb, yb, zb, R_moon, orbit_r);
-  const double ra = sqrt(xa * xa + ya * ya + za * za), rb = sqrt(xb * xb + yb * yb + zb * zb);
-  const double ia = ra > 0 ? 1.0 / ra : 0.0, ib = rb > 0 ? 1.0 / rb : 0.0;
-  c.ax = (float)(xa * ia); c.ay = (float)(ya * ia); c.az = (float)(za * ia);
-  c.bx = (float)(xb * ib); c.by = (float)(yb * ib); c.bz = (float)(zb * ib);
-  // a satellite at the origin sees everything with t = 0 only if s.0 >= 0: always true
-  c.ca = ra > 0 ? (float)(m.ta * ia) : -2.0f;
-  c.cb = rb > 0 ? (float)(m.tb * ib) : -2.0f;
-}
 
 // ------------------------------------------------------------ forward
 // Grid: x = ceil(K / 256) blocks of 256 visibilities (4 wavefronts), y = pixel
@@ -84,84 +45,12 @@ __global__ __launch_bounds__(256) void k_forward(const PixPack* __restrict__ pix
                                                  int64_t K, double R_moon, double orbit_r, int track, int herm,
                                                  double* __restrict__ V) {
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool act = k < K;
-  const int64_t kk = act ? k : K - 1;
-  const double u = uvw[kk], v = uvw[K + kk], w = uvw[2 * K + kk];
-  const double c0 = track ? -w : 0.0;
-  VisMask mk{};
-  VisCls cl{};
-  if (MASK) {
-    make_mask_records(xyza[kk], xyza[K + kk], xyza[2 * K + kk], xyzb[kk], xyzb[K + kk], xyzb[2 * K + kk], R_moon,
-                      orbit_r, mk, cl);
-  }
+  const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, K, k, k < K, R_moon, orbit_r, track);
   const int64_t t0 = (int64_t)blockIdx.y * tiles_per_chunk;
   const int64_t t1 = min(ntile, t0 + tiles_per_chunk);
   double Sr = 0.0, Si = 0.0;
-  for (int64_t t = t0; t < t1; ++t) {
-    bool full = true;
-    if (MASK) {
-      bool any_v, all_v;
-      classify(cone[t], cl, any_v, all_v);
-      any_v = any_v & act;
-      all_v = all_v | !act;
-      if (__ballot(any_v) == 0) continue;
-      full = __ballot(!all_v) == 0;
-    }
-    const PixPack* __restrict__ tp = pix + t * kTile;
-    if (!F64) {
-      float ar0 = 0.f, ai0 = 0.f, ar1 = 0.f, ai1 = 0.f;
-      if (full) {
-#pragma unroll 4
-        for (int j = 0; j < kTile; j += 2) {
-          const PixPack p0 = tp[j], p1 = tp[j + 1];
-          float s0, c0f, s1, c1f;
-          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
-          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
-          ar0 = fmaf(p0.I, c0f, ar0);
-          ai0 = fmaf(p0.I, s0, ai0);
-          ar1 = fmaf(p1.I, c1f, ar1);
-          ai1 = fmaf(p1.I, s1, ai1);
-        }
-      } else {
-#pragma unroll 2
-        for (int j = 0; j < kTile; j += 2) {
-          const PixPack p0 = tp[j], p1 = tp[j + 1];
-          const float I0 = both_visible(p0.l, p0.m, p0.n, mk) ? p0.I : 0.f;
-          const float I1 = both_visible(p1.l, p1.m, p1.n, mk) ? p1.I : 0.f;
-          float s0, c0f, s1, c1f;
-          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
-          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
-          ar0 = fmaf(I0, c0f, ar0);
-          ai0 = fmaf(I0, s0, ai0);
-          ar1 = fmaf(I1, c1f, ar1);
-          ai1 = fmaf(I1, s1, ai1);
-        }
-      }
-      Sr += (double)ar0 + (double)ar1;
-      Si += (double)ai0 + (double)ai1;
-    } else {
-      const double* __restrict__ ip = I64 + t * kTile;
-      for (int j = 0; j < kTile; ++j) {
-        const PixPack p0 = tp[j];
-        double I0 = ip[j];
-        if (!full && !both_visible(p0.l, p0.m, p0.n, mk)) I0 = 0.0;
-        const double f = __builtin_amdgcn_fract(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0))));
-        double s, c;
-        sincospi(2.0 * f, &s, &c);
-        Sr = fma(I0, c, Sr);
-        Si = fma(I0, s, Si);
-      }
-    }
-  }
-  if (act) {
-    // V = sum I exp(-2 pi i ph) = sum I (cos - i sin)
-    unsafeAtomicAdd(&V[2 * k], Sr);
-    unsafeAtomicAdd(&V[2 * k + 1], -Si);
-    if (herm) {
-      unsafeAtomicAdd(&V[2 * (k + K)], Sr);
-      unsafeAtomicAdd(&V[2 * (k + K) + 1], Si);
-    }
-  }
+  forward_tiles<MASK, F64>(pix, I64, cone, nullptr, t0, t1, r, Sr, Si);
+  if (r.act) forward_store(V, k, K, herm, Sr, Si);
 }
 
 // ------------------------------------------------------------ back-projection

@@ -155,3 +155,40 @@ def subset(inp: Inputs, rows: Sequence[int]) -> Inputs:
     rows = np.asarray(rows)
     return Inputs(inp.workload, inp.B, inp.uvw[:, rows].copy(), inp.xyz_a[:, rows].copy(),
                   inp.xyz_b[:, rows].copy(), inp.bll[rows].copy(), inp.positions)
+
+
+# Batched workload: 5 frequencies x 64 snapshot groups x 500 snapshots, 8
+# satellites, nside 128.  Each group flies a short arc (500 snapshots over 2
+# degrees of orbital phase) at its own orbit radius, from 1.01 R to 1.2 R, so
+# the occultation removes a different share of the sky per group (a single
+# occulting sphere can remove at most half of it).
+BATCH_FREQS = (1e6, 2.5e6, 5e6, 7.5e6, 10e6)
+
+
+def batched_workload(nside: int = 128, freqs=BATCH_FREQS, n_groups: int = 64, snaps: int = 500, n_sat: int = 8,
+                     sigma_m: float = 10e3, seed: int = 4, arc_deg: float = 2.0):
+    """(B, uvw, xyz_a, xyz_b, row_offsets, group_freq) with groups ordered frequency-major."""
+    rng = np.random.default_rng(seed)
+    B = make_sky(nside, rng, 20)
+    offsets = rng.normal(0.0, sigma_m, (n_sat, 3))
+    pos_groups = []
+    for g in range(n_groups):
+        r = R_MOON * (1.01 + 0.19 * g / max(1, n_groups - 1))
+        psi = 2 * math.pi * g / n_groups
+        alpha0 = rng.uniform(0, 2 * math.pi)
+        alpha = alpha0 + np.deg2rad(arc_deg) * np.arange(snaps) / snaps
+        e1 = np.array([-math.sin(psi), math.cos(psi), 0.0])
+        e2 = np.array([0.0, 0.0, 1.0])
+        centre = r * (np.cos(alpha)[:, None] * e1 + np.sin(alpha)[:, None] * e2)
+        pos_groups.append(centre[:, None, :] + offsets[None, :, :] * (r / ORBIT_RADIUS))
+    uvws, xas, xbs, offs, gf = [], [], [], [0], []
+    for f in freqs:
+        for pos in pos_groups:
+            uvw, xa, xb, _ = baseline_rows(pos, C_LIGHT / f)
+            uvws.append(uvw)
+            xas.append(xa)
+            xbs.append(xb)
+            offs.append(offs[-1] + uvw.shape[1])
+            gf.append(f)
+    return (B, np.concatenate(uvws, axis=1), np.concatenate(xas, axis=1), np.concatenate(xbs, axis=1),
+            np.array(offs, np.int64), np.array(gf))

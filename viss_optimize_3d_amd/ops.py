@@ -199,3 +199,68 @@ def forward(lmn, I, uvw, xyz_a=None, xyz_b=None, **kw):
 
 def backproject(lmn, uvw, y, wt=None, xyz_a=None, xyz_b=None, **kw):
     return Sky(lmn).backproject(uvw, y, wt, xyz_a, xyz_b, **kw)
+
+
+class Batch:
+    """Many visibility groups over one Sky, forward in one persistent launch
+    (``viss_batch``).  ``row_offsets``: host sequence of n_groups + 1 row
+    offsets into the concatenated [3, K_total] uvw / xyz arrays."""
+
+    def __init__(self, sky: Sky, row_offsets, uvw, xyz_a=None, xyz_b=None, *, mask: bool = True,
+                 R_moon: float = REF_R_MOON, orbit_radius: float = 0.0, phase_track: bool = False,
+                 fp64: bool = False):
+        import numpy as np
+
+        L = _lib.load()
+        self.sky = sky
+        self.uvw, self.xa, self.xb, self.K = sky._rows(uvw, xyz_a, xyz_b, mask)
+        self.offs = np.ascontiguousarray(np.asarray(row_offsets, dtype=np.int64))
+        if self.offs[-1] != self.K:
+            raise ValueError("row_offsets[-1] must equal K_total")
+        self.G = self.offs.size - 1
+        h = ctypes.c_void_p()
+        flags = _flags(mask, phase_track, fp64, False)
+        with torch.cuda.device(sky.device):
+            check("viss_batch_create", L.viss_batch_create(sky._h, self.G, self.offs.ctypes.data_as(ctypes.c_void_p),
+                                                           _p(self.uvw), _p(self.xa), _p(self.xb),
+                                                           _geom(R_moon, orbit_radius), flags, ctypes.byref(h),
+                                                           _stream()))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            torch.cuda.synchronize(self.sky.device)
+            _lib.load().viss_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, I: torch.Tensor, out: Optional[torch.Tensor] = None, use_graph: bool = False) -> torch.Tensor:
+        L = _lib.load()
+        I = _dev(I, torch.float64)
+        if out is None:
+            out = torch.zeros(self.K, dtype=torch.complex128, device=self.sky.device)
+        check("viss_batch_forward", L.viss_batch_forward(self._h, _p(I), _p(out), 1 if use_graph else 0, _stream()))
+        return out
+
+    def autotune(self, I: torch.Tensor, reps: int = 2):
+        L = _lib.load()
+        I = _dev(I, torch.float64)
+        scratch = torch.zeros(self.K, dtype=torch.complex128, device=self.sky.device)
+        check("viss_batch_autotune", L.viss_batch_autotune(self._h, _p(I), _p(scratch), int(reps), _stream()))
+
+    def save_tuning(self, path: str):
+        check("viss_batch_tuning_save", _lib.load().viss_batch_tuning_save(self._h, path.encode()))
+
+    def load_tuning(self, path: str):
+        check("viss_batch_tuning_load", _lib.load().viss_batch_tuning_load(self._h, path.encode(), _stream()))
+
+    def stats(self):
+        vt, tt, ni = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check("viss_batch_stats", _lib.load().viss_batch_stats(self._h, ctypes.byref(vt), ctypes.byref(tt),
+                                                               ctypes.byref(ni)))
+        return {"visible_tiles": vt.value, "total_tiles": tt.value, "n_items": ni.value}
