@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 GPU comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the pix2ang / weights HBM measurements")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched persistent-launch comparison")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first copy a buffer of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -78,6 +80,12 @@ def main():
         sys.exit(2)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calib_copy_mib > 0:
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
+        del src, dst
     wl = geo.WORKLOADS[args.workload]
     log(f"[bench] generating {wl.name}: nside={wl.nside} K={wl.n_vis}")
     inp = geo.generate(wl)
@@ -214,13 +222,15 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
         t1 = time.perf_counter()
         return Vs, t1 - t0
 
-    probe = np.sort(rng.choice(K, 16, replace=False))
-    _, tp = run(probe)
-    per_row = max(tp / probe.size, 1e-6)
-    n_rows = int(min(K, max(16, args.cpu_seconds / 2 / per_row)))
-    rows = np.sort(rng.choice(K, n_rows, replace=False))
-    log(f"[bench] cpu baseline: {n_rows} rows (probe {per_row * 1e3:.2f} ms/row)")
-    Vs, tf = run(rows)
+    # grow the sample until the forward alone takes about half the CPU budget
+    n_rows = 64
+    while True:
+        rows = np.sort(rng.choice(K, n_rows, replace=False))
+        Vs, tf = run(rows)
+        log(f"[bench] cpu baseline: {n_rows} rows forward in {tf:.2f} s")
+        if tf >= args.cpu_seconds / 2 or n_rows >= K:
+            break
+        n_rows = int(min(K, n_rows * max(2.0, min(8.0, (args.cpu_seconds / 2) / max(tf, 1e-3)))))
     # adjoint on the same rows (unit weights) with random y
     ys = rng.normal(size=n_rows) + 1j * rng.normal(size=n_rows)
     t0 = time.perf_counter()

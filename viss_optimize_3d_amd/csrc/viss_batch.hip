@@ -136,10 +136,8 @@ __global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ 
 __global__ void k_reset_counter(unsigned int* c) { *c = 0u; }
 
 template <bool MASK, bool F64>
-__global__ __launch_bounds__(256) void k_forward_batch(const PixPack* __restrict__ pix,
-                                                       const double* __restrict__ I64,
-                                                       const float4* __restrict__ cone,
-                                                       const int32_t* __restrict__ tiles, int64_t ntile,
+__global__ __launch_bounds__(256) void k_forward_batch(const SkyView sv, const int32_t* __restrict__ tiles,
+                                                       int64_t ntile,
                                                        const WorkItem* __restrict__ items, int64_t n_items,
                                                        unsigned int* __restrict__ counter,
                                                        const double* __restrict__ uvw,
@@ -157,7 +155,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const PixPack* __restrict
     const int64_t k = w.row0 + threadIdx.x;
     const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, (int)threadIdx.x < w.nrows, R_moon, orbit_r, track);
     double Sr = 0.0, Si = 0.0;
-    forward_tiles<MASK, F64>(pix, I64, cone, tiles + (int64_t)w.g * ntile, w.j0, w.j1, r, Sr, Si);
+    forward_tiles<MASK, F64>(sv, tiles + (int64_t)w.g * ntile, w.j0, w.j1, r, Sr, Si);
     if (r.act) forward_store(V, k, Ktot, 0, Sr, Si);
   }
 }
@@ -264,7 +262,7 @@ static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, d
   const int track = (b->flags & VISS_PHASE_TRACK) ? 1 : 0;
   const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
 #define VISS_FB(M, F)                                                                                            \
-  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->I64, sky->cone, b->tiles, sky->ntile, items, n_items, \
+  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky_view(sky), b->tiles, sky->ntile, items, n_items, \
                                              b->counter, b->uvw, b->xa, b->xb, b->Ktot, b->R, b->orb, track, V)
   if (mask) {
     if (f64) VISS_FB(true, true); else VISS_FB(true, false);

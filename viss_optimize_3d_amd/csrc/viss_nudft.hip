@@ -38,9 +38,7 @@ namespace viss {
 // Grid: x = ceil(K / 256) blocks of 256 visibilities (4 wavefronts), y = pixel
 // chunks of `tiles_per_chunk` 64-pixel tiles.
 template <bool MASK, bool F64>
-__global__ __launch_bounds__(256) void k_forward(const PixPack* __restrict__ pix, const double* __restrict__ I64,
-                                                 const float4* __restrict__ cone, int64_t ntile,
-                                                 int64_t tiles_per_chunk, const double* __restrict__ uvw,
+__global__ __launch_bounds__(256) void k_forward(const SkyView sv, int64_t ntile, int64_t tiles_per_chunk, const double* __restrict__ uvw,
                                                  const double* __restrict__ xyza, const double* __restrict__ xyzb,
                                                  int64_t K, double R_moon, double orbit_r, int track, int herm,
                                                  double* __restrict__ V) {
@@ -49,7 +47,7 @@ __global__ __launch_bounds__(256) void k_forward(const PixPack* __restrict__ pix
   const int64_t t0 = (int64_t)blockIdx.y * tiles_per_chunk;
   const int64_t t1 = min(ntile, t0 + tiles_per_chunk);
   double Sr = 0.0, Si = 0.0;
-  forward_tiles<MASK, F64>(pix, I64, cone, nullptr, t0, t1, r, Sr, Si);
+  forward_tiles<MASK, F64>(sv, nullptr, t0, t1, r, Sr, Si);
   if (r.act) forward_store(V, k, K, herm, Sr, Si);
 }
 
@@ -268,7 +266,7 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
 #define VISS_FWD(M, F)                                                                                          \
-  k_forward<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->I64, sky->cone, sky->ntile, tpc, uvw, xyza, xyzb, K, R, orb, \
+  k_forward<M, F><<<grid, 256, 0, s>>>(sky_view(sky), sky->ntile, tpc, uvw, xyza, xyzb, K, R, orb, \
                                        track, herm, V)
   if (mask) {
     if (f64) VISS_FWD(true, true); else VISS_FWD(true, false);
