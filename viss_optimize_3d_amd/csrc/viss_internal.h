@@ -105,8 +105,6 @@ struct viss_sky {
   int64_t nbtile = 0;  // P_pad / kBpTile
   viss::PixPack* pix = nullptr;  // [P_pad] sorted
   double* I64 = nullptr;         // [P_pad] FP64 brightness (all-FP64 forward)
-  double* sl = nullptr;          // [3][P_pad] SoA directions in sorted order (forward tiles)
-  float* sI = nullptr;           // [P_pad] FP32 brightness in sorted order (mixed forward)
   float4* cone = nullptr;        // [ntile] bounding cones of kTile-pixel tiles
   float4* bcone = nullptr;       // [nbtile] bounding cones of kBpTile-pixel tiles
   int32_t* tile_valid = nullptr; // [ntile] number of real pixels per tile

@@ -73,106 +73,75 @@ __device__ __forceinline__ FwdRow load_fwd_row(const double* __restrict__ uvw, c
   return r;
 }
 
-// Forward view of a sky: SoA directions and brightness in sorted order, so a
-// group of consecutive pixels of one tile is one wide scalar load per field.
+// Forward view of a sky: the sorted pixel pack, FP64 brightness and tile cones.
 struct SkyView {
-  const double* sl;  // [3][P_pad]
-  int64_t P_pad;
-  const float* sI;   // [P_pad]
-  const double* I64; // [P_pad]
+  const PixPack* pix;
+  const double* I64;
   const float4* cone;
 };
 
-__host__ inline SkyView sky_view(const viss_sky* s) { return SkyView{s->sl, s->P_pad, s->sI, s->I64, s->cone}; }
+__host__ inline SkyView sky_view(const viss_sky* s) { return SkyView{s->pix, s->I64, s->cone}; }
 
-// One tile, mixed precision: sum_p I_p (cos, sin)(2 pi phase) over its 64
-// pixels, software-pipelined four pixels ahead (one scalar-load wait covers
-// four pixels).  CHECK: exact FP64 occultation test per pair.
-template <bool CHECK>
-__device__ __forceinline__ void tile_mixed(const double* __restrict__ L, const double* __restrict__ M,
-                                           const double* __restrict__ N, const float* __restrict__ Iv,
-                                           const FwdRow& r, double& Sr, double& Si) {
-  constexpr int G = 4;
-  const double u = r.u, v = r.v, w = r.w, c0 = r.c0;
-  double cl[G], cm[G], cn[G];
-  float ci[G];
-#pragma unroll
-  for (int q = 0; q < G; ++q) {
-    cl[q] = L[q];
-    cm[q] = M[q];
-    cn[q] = N[q];
-    ci[q] = Iv[q];
-  }
-  float ar0 = 0.f, ai0 = 0.f, ar1 = 0.f, ai1 = 0.f;
-  for (int j = 0; j < kTile; j += G) {
-    const int jn = (j + G) & (kTile - 1);  // the last group reloads the first: in bounds, unused
-    double nl[G], nm[G], nn[G];
-    float ni[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      nl[q] = L[jn + q];
-      nm[q] = M[jn + q];
-      nn[q] = N[jn + q];
-      ni[q] = Iv[jn + q];
-    }
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      float I0 = ci[q];
-      if (CHECK && !both_visible(cl[q], cm[q], cn[q], r.mk)) I0 = 0.f;
-      float s0, cs;
-      sincos_rev(rev_fp32(fma(u, cl[q], fma(v, cm[q], fma(w, cn[q], c0)))), s0, cs);
-      if (q & 1) {
-        ar1 = fmaf(I0, cs, ar1);
-        ai1 = fmaf(I0, s0, ai1);
-      } else {
-        ar0 = fmaf(I0, cs, ar0);
-        ai0 = fmaf(I0, s0, ai0);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      cl[q] = nl[q];
-      cm[q] = nm[q];
-      cn[q] = nn[q];
-      ci[q] = ni[q];
-    }
-  }
-  Sr += (double)ar0 + (double)ar1;
-  Si += (double)ai0 + (double)ai1;
-}
-
-// Accumulates over the tiles t = list[j] (t = j when list is NULL), j in
-// [j0, j1), into (Sr, Si).  All lanes of a wavefront walk the same tiles.
+// Accumulates sum_p I_p (cos, sin)(2 pi phase) over the tiles t = list[j]
+// (t = j when list is NULL), j in [j0, j1), into (Sr, Si).  All lanes of a
+// wavefront walk the same tiles; tile data are wave-uniform.
 template <bool MASK, bool F64>
 __device__ __forceinline__ void forward_tiles(const SkyView& sv, const int32_t* __restrict__ list, int64_t j0,
                                               int64_t j1, const FwdRow& r, double& Sr, double& Si) {
+  const PixPack* __restrict__ pix = sv.pix;
+  const double* __restrict__ I64 = sv.I64;
+  const float4* __restrict__ cone = sv.cone;
   const double u = r.u, v = r.v, w = r.w, c0 = r.c0;
   for (int64_t jt = j0; jt < j1; ++jt) {
     const int64_t t = list ? (int64_t)list[jt] : jt;
     bool full = true;
     if (MASK) {
       bool any_v, all_v;
-      classify(sv.cone[t], r.cl, any_v, all_v);
+      classify(cone[t], r.cl, any_v, all_v);
       any_v = any_v & r.act;
       all_v = all_v | !r.act;
       if (__ballot(any_v) == 0) continue;
       full = __ballot(!all_v) == 0;
     }
-    const double* __restrict__ L = sv.sl + t * kTile;
-    const double* __restrict__ M = sv.sl + sv.P_pad + t * kTile;
-    const double* __restrict__ N = sv.sl + 2 * sv.P_pad + t * kTile;
+    const PixPack* __restrict__ tp = pix + t * kTile;
     if (!F64) {
-      const float* __restrict__ Iv = sv.sI + t * kTile;
-      if (!MASK || full)
-        tile_mixed<false>(L, M, N, Iv, r, Sr, Si);
-      else
-        tile_mixed<true>(L, M, N, Iv, r, Sr, Si);
+      float ar0 = 0.f, ai0 = 0.f, ar1 = 0.f, ai1 = 0.f;
+      if (full) {
+#pragma unroll 4
+        for (int j = 0; j < kTile; j += 2) {
+          const PixPack p0 = tp[j], p1 = tp[j + 1];
+          float s0, c0f, s1, c1f;
+          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
+          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
+          ar0 = fmaf(p0.I, c0f, ar0);
+          ai0 = fmaf(p0.I, s0, ai0);
+          ar1 = fmaf(p1.I, c1f, ar1);
+          ai1 = fmaf(p1.I, s1, ai1);
+        }
+      } else {
+#pragma unroll 2
+        for (int j = 0; j < kTile; j += 2) {
+          const PixPack p0 = tp[j], p1 = tp[j + 1];
+          const float I0 = both_visible(p0.l, p0.m, p0.n, r.mk) ? p0.I : 0.f;
+          const float I1 = both_visible(p1.l, p1.m, p1.n, r.mk) ? p1.I : 0.f;
+          float s0, c0f, s1, c1f;
+          sincos_rev(rev_fp32(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0)))), s0, c0f);
+          sincos_rev(rev_fp32(fma(u, p1.l, fma(v, p1.m, fma(w, p1.n, c0)))), s1, c1f);
+          ar0 = fmaf(I0, c0f, ar0);
+          ai0 = fmaf(I0, s0, ai0);
+          ar1 = fmaf(I1, c1f, ar1);
+          ai1 = fmaf(I1, s1, ai1);
+        }
+      }
+      Sr += (double)ar0 + (double)ar1;
+      Si += (double)ai0 + (double)ai1;
     } else {
-      const double* __restrict__ ip = sv.I64 + t * kTile;
+      const double* __restrict__ ip = I64 + t * kTile;
       for (int j = 0; j < kTile; ++j) {
+        const PixPack p0 = tp[j];
         double I0 = ip[j];
-        if (!full && !both_visible(L[j], M[j], N[j], r.mk)) I0 = 0.0;
-        const double f = __builtin_amdgcn_fract(fma(u, L[j], fma(v, M[j], fma(w, N[j], c0))));
+        if (!full && !both_visible(p0.l, p0.m, p0.n, r.mk)) I0 = 0.0;
+        const double f = __builtin_amdgcn_fract(fma(u, p0.l, fma(v, p0.m, fma(w, p0.n, c0))));
         double s, c;
         sincospi(2.0 * f, &s, &c);
         Sr = fma(I0, c, Sr);

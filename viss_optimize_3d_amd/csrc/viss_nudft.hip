@@ -30,6 +30,8 @@
 //    pixel / visibility chunks meet through FP64 atomics in the caller's
 //    accumulate-into buffer.
 //  * VISS_PREC_FP64: same structure, FP64 sincospi and FP64 accumulation.
+#include <cstdlib>
+
 #include "viss_kernels.h"
 
 namespace viss {
@@ -234,6 +236,17 @@ static int64_t choose_chunks(int64_t waves_x, int64_t units, int64_t min_chunk) 
   return ch < 1 ? 1 : ch;
 }
 
+// Launch-shape knobs: VISS_FWD_TILES (64-pixel tiles per forward block) and
+// VISS_BP_ROWS (visibility rows per back-projection block).  Blocks resident
+// at the same time walk the same chunk, so a chunk that fits in one XCD's L2
+// keeps the wave-uniform pixel / visibility reads on chip.
+static int64_t env_int(const char* name, int64_t dflt) {
+  const char* v = getenv(name);
+  if (!v || !*v) return dflt;
+  const long long x = atoll(v);
+  return x > 0 ? (int64_t)x : dflt;
+}
+
 static int validate_geom(const viss_mask_geometry* g, int flags) {
   if (flags & VISS_MASK) {
     VISS_REQUIRE(g != nullptr, "VISS_MASK needs a viss_mask_geometry");
@@ -260,9 +273,10 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   if (rc != VISS_OK) return rc;
   const double R = mask ? geom->R_moon : 0.0, orb = mask ? geom->orbit_radius : 0.0;
   const int64_t bx = cdiv(K, 256);
-  const int64_t chunks = choose_chunks(bx * 4, sky->ntile, 64);
-  const int64_t tpc = cdiv(sky->ntile, chunks);
+  int64_t tpc = env_int("VISS_FWD_TILES", 0);
+  if (tpc <= 0) tpc = cdiv(sky->ntile, choose_chunks(bx * 4, sky->ntile, 64));
   VISS_REQUIRE(bx <= INT32_MAX, "viss_forward: K too large for one launch");
+  VISS_REQUIRE(cdiv(sky->ntile, tpc) <= 65535, "viss_forward: too many pixel chunks");
   dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
 #define VISS_FWD(M, F)                                                                                          \
@@ -308,8 +322,10 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
     VISS_LAUNCHED(k_bp_prologue);
   }
   const int64_t bx = cdiv(sky->nbtile, 4);
-  const int64_t chunks = choose_chunks(sky->nbtile, cdiv(K, 64), 16);
-  int64_t kpc = cdiv(cdiv(K, chunks), 64) * 64;
+  int64_t kpc = env_int("VISS_BP_ROWS", 0);
+  if (kpc <= 0) kpc = cdiv(K, choose_chunks(sky->nbtile, cdiv(K, 64), 16));
+  kpc = cdiv(kpc, 64) * 64;
+  if (cdiv(K, kpc) > 65535) kpc = cdiv(cdiv(K, 65535), 64) * 64;
   dim3 grid((unsigned)bx, (unsigned)cdiv(K, kpc));
   const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
 #define VISS_BP(M, F) \

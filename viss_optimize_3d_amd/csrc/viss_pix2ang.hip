@@ -16,35 +16,35 @@
 
 namespace viss {
 
-__device__ inline int64_t compact_even_bits(uint64_t v) {
-  v &= 0x5555555555555555ull;
-  v = (v | (v >> 1)) & 0x3333333333333333ull;
-  v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
-  v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
-  v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
-  v = (v | (v >> 16)) & 0x00000000ffffffffull;
-  return (int64_t)v;
+// All indices fit in 32 bits for nside <= 8192 (npix < 2^30), so the integer
+// work uses 32-bit operations; 64-bit division would cost several times more.
+__device__ inline uint32_t compact_even_bits(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0f0f0f0fu;
+  v = (v | (v >> 4)) & 0x00ff00ffu;
+  v = (v | (v >> 8)) & 0x0000ffffu;
+  return v;
 }
 
-__device__ inline int64_t isqrt64(int64_t x) {
-  int64_t s = (int64_t)sqrt((double)x);
-  while (s * s > x) --s;
-  while ((s + 1) * (s + 1) <= x) ++s;
+__device__ inline int32_t isqrt32(int32_t x) {
+  int32_t s = (int32_t)sqrt((double)x);
+  while ((int64_t)s * s > x) --s;
+  while ((int64_t)(s + 1) * (s + 1) <= x) ++s;
   return s;
 }
 
 struct RingPos {
-  int64_t jr, jp, nr, kshift;
+  int32_t jr, jp, nr, kshift;
 };
 
 // NEST: face / in-face (ix, iy) -> ring jr and in-ring jp (pix2ang.cu:99-157,
-// 0-based tables jrll/jpll of pix2ang.cu:183-184).
-__device__ inline RingPos nest_pos(int64_t ip, int64_t nside) {
+// 0-based tables jrll/jpll of pix2ang.cu:183-184).  nside = 2^order.
+__device__ inline RingPos nest_pos(int32_t ip, int32_t nside, int32_t order) {
   constexpr int8_t jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
   constexpr int8_t jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
-  const int64_t npface = nside * nside;
-  const int64_t face = ip / npface, ipf = ip - face * npface;
-  const int64_t ix = compact_even_bits((uint64_t)ipf), iy = compact_even_bits((uint64_t)ipf >> 1);
+  const int32_t face = ip >> (2 * order), ipf = ip & ((1 << (2 * order)) - 1);
+  const int32_t ix = (int32_t)compact_even_bits((uint32_t)ipf), iy = (int32_t)compact_even_bits((uint32_t)ipf >> 1);
   RingPos r;
   r.jr = jrll[face] * nside - (ix + iy) - 1;
   if (r.jr < nside) {
@@ -57,7 +57,7 @@ __device__ inline RingPos nest_pos(int64_t ip, int64_t nside) {
     r.nr = 4 * nside - r.jr;
     r.kshift = 0;
   }
-  int64_t jp = (jpll[face] * r.nr + (ix - iy) + 1 + r.kshift) / 2;
+  int32_t jp = (jpll[face] * r.nr + (ix - iy) + 1 + r.kshift) / 2;
   if (jp > 4 * nside) jp -= 4 * nside;
   if (jp < 1) jp += 4 * nside;
   r.jp = jp;
@@ -65,24 +65,25 @@ __device__ inline RingPos nest_pos(int64_t ip, int64_t nside) {
 }
 
 // RING: Gorski et al. 2005 sec. 4 (polar caps / equatorial belt).
-__device__ inline RingPos ring_pos(int64_t p, int64_t nside) {
-  const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1), nl4 = 4 * nside;
+__device__ inline RingPos ring_pos(int32_t p, int32_t nside) {
+  const int32_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1), nl4 = 4 * nside;
   RingPos r;
   r.kshift = 0;
   if (p < ncap) {
-    const int64_t i = (1 + isqrt64(1 + 2 * p)) / 2;
+    const int32_t i = (1 + isqrt32(1 + 2 * p)) / 2;
     r.jr = i;
     r.nr = i;
     r.jp = p + 1 - 2 * i * (i - 1);
   } else if (p < npix - ncap) {
-    const int64_t q = p - ncap;
-    r.jr = q / nl4 + nside;
-    r.jp = q % nl4 + 1;
+    const uint32_t q = (uint32_t)(p - ncap);
+    const uint32_t row = q / (uint32_t)nl4;
+    r.jr = (int32_t)row + nside;
+    r.jp = (int32_t)(q - row * (uint32_t)nl4) + 1;
     r.nr = nside;
     r.kshift = (r.jr - nside) & 1;
   } else {
-    const int64_t q = npix - p;
-    const int64_t i = (1 + isqrt64(2 * q - 1)) / 2;
+    const int32_t q = npix - p;
+    const int32_t i = (1 + isqrt32(2 * q - 1)) / 2;
     r.jr = nl4 - i;
     r.nr = i;
     r.jp = 4 * i + 1 - (q - 2 * i * (i - 1));
@@ -91,7 +92,7 @@ __device__ inline RingPos ring_pos(int64_t p, int64_t nside) {
 }
 
 #pragma clang fp contract(off)
-__device__ inline void ring_angles(const RingPos& r, int64_t nside, double& z, double& phi) {
+__device__ inline void ring_angles(const RingPos& r, int32_t nside, double& z, double& phi) {
   const double three_ns2 = 3.0 * (double)(nside * nside);
   if (r.jr < nside) {
     z = (three_ns2 - (double)(r.nr * r.nr)) / three_ns2;
@@ -103,21 +104,20 @@ __device__ inline void ring_angles(const RingPos& r, int64_t nside, double& z, d
   phi = (kPi / 2.0) * ((double)r.jp - 0.5 * (double)(r.kshift + 1)) / (double)r.nr;
 }
 
-// lmn (optional) is computed from theta, phi exactly as the pre-pass does
-// (vissgen3d_noblockage.cu:58-70): l = sin t cos p, m = -sin t sin p, n = cos t.
-__global__ void k_pix2ang(int64_t nside, int nest, int64_t npix, double* __restrict__ theta, double* __restrict__ phi,
-                          int32_t* __restrict__ ring, int32_t* __restrict__ iphi) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
-    const RingPos r = nest ? nest_pos(p, nside) : ring_pos(p, nside);
+__global__ void k_pix2ang(int32_t nside, int32_t order, int32_t npix, double* __restrict__ theta,
+                          double* __restrict__ phi, int32_t* __restrict__ ring, int32_t* __restrict__ iphi) {
+  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    const RingPos r = order >= 0 ? nest_pos(p, nside, order) : ring_pos(p, nside);
     double z, ph;
     ring_angles(r, nside, z, ph);
     theta[p] = acos(z);
     phi[p] = ph;
-    if (ring) ring[p] = (int32_t)r.jr;
-    if (iphi) iphi[p] = (int32_t)r.jp;
+    if (ring) ring[p] = r.jr;
+    if (iphi) iphi[p] = r.jp;
   }
 }
 
+// l = sin t cos p, m = -sin t sin p, n = cos t (vissgen3d_noblockage.cu:58-70).
 __global__ void k_pixel_dirs(const double* __restrict__ theta, const double* __restrict__ phi,
                              const double* __restrict__ B, int64_t P, double scale, double* __restrict__ lmn,
                              double* __restrict__ I) {
@@ -152,7 +152,12 @@ extern "C" int viss_pix2ang(int64_t nside, int scheme, double* theta, double* ph
   VISS_REQUIRE(scheme == VISS_SCHEME_RING || (nside & (nside - 1)) == 0,
                "viss_pix2ang: NEST needs a power-of-two nside (got %lld)", (long long)nside);
   const int64_t npix = 12 * nside * nside;
-  k_pix2ang<<<grid_1d(npix), 256, 0, as_stream(stream)>>>(nside, scheme == VISS_SCHEME_NEST, npix, theta, phi, ring,
+  int32_t order = -1;
+  if (scheme == VISS_SCHEME_NEST) {
+    order = 0;
+    while ((1ll << order) < nside) ++order;
+  }
+  k_pix2ang<<<grid_1d(npix), 256, 0, as_stream(stream)>>>((int32_t)nside, order, (int32_t)npix, theta, phi, ring,
                                                            iphi);
   VISS_LAUNCHED(k_pix2ang);
   return VISS_OK;

@@ -77,8 +77,7 @@ __global__ void k_morton_keys(const double* __restrict__ lmn, int64_t P, uint32_
 }
 
 __global__ void k_pack_sky(const double* __restrict__ lmn, int64_t P, int64_t P_pad, const int32_t* __restrict__ order,
-                           PixPack* __restrict__ pix, double* __restrict__ I64, double* __restrict__ sl,
-                           float* __restrict__ sI) {
+                           PixPack* __restrict__ pix, double* __restrict__ I64) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P_pad; i += (int64_t)gridDim.x * blockDim.x) {
     PixPack q;
     if (i < P) {
@@ -94,10 +93,6 @@ __global__ void k_pack_sky(const double* __restrict__ lmn, int64_t P, int64_t P_
     q.I = 0.0f;
     pix[i] = q;
     I64[i] = 0.0;
-    sl[i] = q.l;
-    sl[P_pad + i] = q.m;
-    sl[2 * P_pad + i] = q.n;
-    sI[i] = 0.0f;
   }
 }
 
@@ -159,12 +154,11 @@ __global__ void k_tile_cones(const PixPack* __restrict__ pix, int64_t ntiles, in
 }
 
 __global__ void k_scatter_brightness(PixPack* __restrict__ pix, int64_t P_pad, const double* __restrict__ I,
-                                     double* __restrict__ I64, float* __restrict__ sI, int want64) {
+                                     double* __restrict__ I64, int want64) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P_pad; i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t o = pix[i].idx;
     const double v = o >= 0 ? I[o] : 0.0;
     pix[i].I = (float)v;
-    sI[i] = (float)v;
     if (want64) I64[i] = v;
   }
 }
@@ -214,9 +208,7 @@ int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
   VISS_HIP(hipMalloc((void**)&sky->cone, sizeof(float4) * sky->ntile));
   VISS_HIP(hipMalloc((void**)&sky->bcone, sizeof(float4) * sky->nbtile));
   VISS_HIP(hipMalloc((void**)&sky->tile_valid, sizeof(int32_t) * sky->ntile));
-  VISS_HIP(hipMalloc((void**)&sky->sl, sizeof(double) * 3 * Pp));
-  VISS_HIP(hipMalloc((void**)&sky->sI, sizeof(float) * Pp));
-  k_pack_sky<<<grid_for(Pp, 256), 256, 0, s>>>(lmn, P, Pp, v_out, sky->pix, sky->I64, sky->sl, sky->sI);
+  k_pack_sky<<<grid_for(Pp, 256), 256, 0, s>>>(lmn, P, Pp, v_out, sky->pix, sky->I64);
   VISS_LAUNCHED(k_pack_sky);
   k_tile_cones<<<(unsigned)((sky->ntile + 3) / 4), 256, 0, s>>>(sky->pix, sky->ntile, kTile / 64, sky->cone,
                                                                   sky->tile_valid);
@@ -239,10 +231,6 @@ void sky_free(viss_sky* sky) {
   if (sky->cone) (void)hipFree(sky->cone);
   if (sky->bcone) (void)hipFree(sky->bcone);
   if (sky->tile_valid) (void)hipFree(sky->tile_valid);
-  if (sky->sl) (void)hipFree(sky->sl);
-  if (sky->sI) (void)hipFree(sky->sI);
-  sky->sl = nullptr;
-  sky->sI = nullptr;
   sky->ws.release();
   sky->pix = nullptr;
   sky->I64 = nullptr;
@@ -251,8 +239,7 @@ void sky_free(viss_sky* sky) {
 }
 
 int sky_set_brightness(viss_sky* sky, const double* I, bool want64, hipStream_t s) {
-  k_scatter_brightness<<<grid_for(sky->P_pad, 256), 256, 0, s>>>(sky->pix, sky->P_pad, I, sky->I64, sky->sI,
-                                                                  want64 ? 1 : 0);
+  k_scatter_brightness<<<grid_for(sky->P_pad, 256), 256, 0, s>>>(sky->pix, sky->P_pad, I, sky->I64, want64 ? 1 : 0);
   VISS_LAUNCHED(k_scatter_brightness);
   return VISS_OK;
 }
