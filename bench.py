@@ -59,13 +59,62 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 GPU comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the pix2ang / weights HBM measurements")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched persistent-launch comparison")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the HBM-traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
+def _pmc_rows(d):
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(d, "*counter_collection.csv"))):
+        rows += list(csv.DictReader(open(f)))
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
+    """HBM bytes per dispatch of the hot kernels from FETCH_SIZE / WRITE_SIZE (KiB),
+    each scaled by what the counter reads for a plain copy of calib_mib MiB."""
+    fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    kib = calib_mib * 1024.0
+    copies_w = [float(r["Counter_Value"]) for r in wr if "copyBuffer" in r["Kernel_Name"]]
+    copies_f = [float(r["Counter_Value"]) for r in fr if "copyBuffer" in r["Kernel_Name"]]
+    fw = kib / max(copies_w)  # the calibration copy is the largest copy of the run
+    ff = kib / max(copies_f)
+    out = [f"calibration: {calib_mib} MiB copy reads FETCH_SIZE={max(copies_f):.1f} KiB, WRITE_SIZE={max(copies_w):.1f} KiB",
+           f"scale factors: fetch x{ff:.4f}, write x{fw:.4f}"]
+    for key in ("k_forward<true, false>", "k_backproject<true, false>", "k_pix2ang", "k_shell_hist", "k_row_weights"):
+        f = [float(r["Counter_Value"]) for r in fr if key in r["Kernel_Name"]]
+        w = [float(r["Counter_Value"]) for r in wr if key in r["Kernel_Name"]]
+        if not f or not w:
+            continue
+        fb, wb = f[0] * 1024 * ff, w[0] * 1024 * fw
+        out.append(f"kernel={key.split('<')[0]} fetch_bytes={fb:.6e} write_bytes={wb:.6e} bytes_per_launch={fb + wb:.6e}")
+    return "\n".join(out)
+
+
+def committed_traffic(kernel, workload):
+    """bytes_per_launch of `kernel` from profiles/pmc_traffic_<workload>.txt (None if absent)."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"kernel={kernel} "):
+            for tok in line.split():
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=")[1])
+    return None
+
+
 def main():
     args = parse()
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -162,7 +211,7 @@ def main():
     achieved = flops / (dom_ms * 1e-3)
     peak = (PAIR_FP32 + PAIR_FP64) / PAIR_T_PEAK
     out["roofline"] = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
-                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": committed_traffic(dom, wl.name),
                        "flops_per_pair": {"fp32": PAIR_FP32, "fp64": PAIR_FP64}, "pairs_per_launch": visible}
     out["detail"] = {
         "visible_pairs_per_pass": visible, "dense_pairs_per_pass": dense, "visible_fraction": visible / dense,

@@ -27,6 +27,11 @@ def rel(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
+def zm_scale(a, b):
+    a0, b0 = a - a.mean(), b - b.mean()
+    return float(torch.dot(a0, b0) / torch.dot(a0, a0))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="diff-nside128")
@@ -92,6 +97,9 @@ def main(argv=None):
         "cg_residual_first_last": [hist[0], hist[-1]], "cg_seconds": t_cg,
         "cg_rel_err": rel(x, x_true), "cg_rel_err_scaled": rel(beta * x, x_true),
         "dirty_rel_err": rel(dirty, x_true), "dirty_rel_err_scaled": rel(alpha * dirty, x_true),
+        # the monopole (sky mean) is barely constrained by non-zero baselines: errors without it
+        "cg_rel_err_zero_mean": rel(x - x.mean(), x_true - x_true.mean()),
+        "dirty_rel_err_zero_mean_scaled": rel(zm_scale(dirty, x_true) * (dirty - dirty.mean()), x_true - x_true.mean()),
     }
     print(json.dumps(out), flush=True)
     return out
