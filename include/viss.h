@@ -168,6 +168,17 @@ int viss_batch_tuning_save(const viss_batch* batch, const char* path);
 int viss_batch_tuning_load(viss_batch* batch, const char* path, void* stream);
 int viss_batch_stats(const viss_batch* batch, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items);
 
+/* ---- image utilities after the hot path ----
+ * Multi-day aggregation, the notebooks' sum over days
+ * (healpix3d_blockage.ipynb: total += np.nan_to_num(C_i);
+ * healpix3d_noblockage.ipynb: total += C_i):
+ *   total[p] += (nan_to_zero && isnan(img[p])) ? 0 : img[p]. */
+int viss_image_accumulate(const double* img, int64_t P, int nan_to_zero, double* total, void* stream);
+/* HEALPix scheme reorder of a map of 12 nside^2 FP64 values (nside a power of
+ * two): to_ring != 0 maps a NEST-ordered map to RING order, else RING to NEST
+ * (the notebooks display NEST maps, hp.mollview(nest=True)). in != out. */
+int viss_reorder(const double* in, int64_t nside, int to_ring, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

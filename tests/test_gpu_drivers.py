@@ -89,6 +89,19 @@ def test_pix2ang_and_vissgen3d_end_to_end(tmp_path, variant):
         dflt = np.loadtxt(outdir / f"C{day}day1M.txt")
         assert rel_l2(dflt, ref) < 1e-5
         assert np.isfinite(dflt).all()
+    # multi-day aggregation (the notebooks' sum), from the text files and from the binary sidecars
+    run(["vissgen3d", "--variant", variant, "--days", "2", "--binary", "--out", "withbin"], cwd=str(tmp_path))
+    run(["vissimage_sum", "--dir", str(outdir), "--days", "2", "--nan-to-zero", "--precision", "17", "--out",
+         "sum_txt.txt"], cwd=str(tmp_path))
+    run(["vissimage_sum", "--dir", "withbin", "--days", "2", "--binary", "--precision", "17", "--out", "sum_bin.txt",
+         "--to-ring", str(nside)], cwd=str(tmp_path))
+    s_txt = np.loadtxt(tmp_path / "sum_txt.txt")
+    np.testing.assert_array_equal(s_txt, np.loadtxt(outdir / "C1day1M.txt") + np.loadtxt(outdir / "C2day1M.txt"))
+    b1 = np.fromfile(tmp_path / "withbin" / "C1day1M.bin", dtype=np.float64)
+    b2 = np.fromfile(tmp_path / "withbin" / "C2day1M.bin", dtype=np.float64)
+    ring = np.empty(P)
+    ring[oracle.nest2ring(nside, np.arange(P))] = b1 + b2
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "sum_bin.txt"), ring)
 
 
 def test_vissgen3d_missing_day_files_give_zero_image(tmp_path):

@@ -27,7 +27,9 @@
 //     reference) extend nr instead (--no-extend-nr: weight 0);
 //   * dcf uses the noblockage indexing in every variant
 //     (--dcf-blockage-shift: vissgen3d_blockage.cu:138);
-//   * all arithmetic on inputs is FP64 (phase) / mixed (--fp64: all FP64).
+//   * all arithmetic on inputs is FP64 (phase) / mixed (--fp64: all FP64);
+//   * --binary also writes each image as raw FP64 (C{p}day1M.bin) for fast
+//     aggregation (vissimage_sum --binary).
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 #include <sys/time.h>
@@ -70,7 +72,7 @@ struct Options {
   double R_moon = 1737.1e3, altitude = 300e3;  // vissgen3d_noblockage.cu:240-241
   bool per_sat_cut = false;
   double bl_max = 100e3;  // :248
-  bool dcf_shift = false, extend_nr = true, fp64 = false, resume = false;
+  bool dcf_shift = false, extend_nr = true, fp64 = false, resume = false, binary = false;
   int image_mask = -1;  // -1: per variant
   int precision = 0;
   int device = 0;
@@ -82,7 +84,7 @@ int usage() {
           "                 [--sky FILE] [--theta FILE] [--phi FILE] [--freq HZ] [--days N] [--first-day D]\n"
           "                 [--lambda-c] [--R-moon M] [--altitude M] [--per-satellite-cut] [--bl-max M]\n"
           "                 [--dcf-blockage-shift] [--no-extend-nr] [--image-mask on|off] [--fp64]\n"
-          "                 [--precision P] [--resume] [--device D]\n");
+          "                 [--precision P] [--binary] [--resume] [--device D]\n");
   return 2;
 }
 
@@ -149,6 +151,7 @@ int main(int argc, char** argv) {
     else if (a == "--fp64") o.fp64 = true;
     else if (a == "--precision" && (v = next())) o.precision = atoi(v);
     else if (a == "--resume") o.resume = true;
+    else if (a == "--binary") o.binary = true;
     else if (a == "--device" && (v = next())) o.device = atoi(v);
     else return usage();
   }
@@ -342,6 +345,16 @@ int main(int argc, char** argv) {
     if (!vissio::write_column(out_path, hC.data(), P, o.precision)) {
       fprintf(stderr, "vissgen3d: cannot write %s\n", out_path.c_str());
       return 1;
+    }
+    if (o.binary) {  // raw little-endian FP64 sidecar, C{p}day1M.bin
+      const std::string bin_path = o.out_dir + "C" + std::to_string(day) + "day1M.bin";
+      FILE* f = fopen(bin_path.c_str(), "wb");
+      if (!f || fwrite(hC.data(), sizeof(double), P, f) != (size_t)P) {
+        fprintf(stderr, "vissgen3d: cannot write %s\n", bin_path.c_str());
+        if (f) fclose(f);
+        return 1;
+      }
+      fclose(f);
     }
     printf("day %d: saved %s\n", day, out_path.c_str());
   }

@@ -264,3 +264,21 @@ class Batch:
         check("viss_batch_stats", _lib.load().viss_batch_stats(self._h, ctypes.byref(vt), ctypes.byref(tt),
                                                                ctypes.byref(ni)))
         return {"visible_tiles": vt.value, "total_tiles": tt.value, "n_items": ni.value}
+
+
+def image_accumulate(img: torch.Tensor, total: torch.Tensor, nan_to_zero: bool = True) -> torch.Tensor:
+    """total += nan_to_num(img) (the notebooks' multi-day sum), in place."""
+    img = _dev(img, torch.float64)
+    if total.dtype != torch.float64 or not total.is_contiguous() or total.numel() != img.numel():
+        raise ValueError("total must be a contiguous float64 tensor of the image's size")
+    check("viss_image_accumulate", _lib.load().viss_image_accumulate(_p(img), img.numel(), 1 if nan_to_zero else 0,
+                                                                     _p(total), _stream()))
+    return total
+
+
+def reorder(m: torch.Tensor, nside: int, to_ring: bool) -> torch.Tensor:
+    """NEST -> RING (to_ring) or RING -> NEST reorder of a HEALPix map."""
+    m = _dev(m, torch.float64)
+    out = torch.empty_like(m)
+    check("viss_reorder", _lib.load().viss_reorder(_p(m), int(nside), 1 if to_ring else 0, _p(out), _stream()))
+    return out
