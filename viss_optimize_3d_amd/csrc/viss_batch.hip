@@ -230,9 +230,12 @@ static void build_items(const viss_batch* b, const std::vector<int>& groups, std
         out.push_back(w);
       }
   }
-  // largest items first: the atomic queue then ends on small ones
+  // Items that run at the same time should read the same tiles: order by the
+  // position in the (tile-sorted) lists, then by group, so the queue sweeps the
+  // sky once and concurrent items share L2.
   std::stable_sort(out.begin(), out.end(), [](const WorkItem& a, const WorkItem& c) {
-    return (int64_t)a.nrows * (a.j1 - a.j0) > (int64_t)c.nrows * (c.j1 - c.j0);
+    if (a.j0 != c.j0) return a.j0 < c.j0;
+    return a.g < c.g;
   });
 }
 

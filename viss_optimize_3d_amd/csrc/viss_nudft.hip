@@ -30,6 +30,7 @@
 //    pixel / visibility chunks meet through FP64 atomics in the caller's
 //    accumulate-into buffer.
 //  * VISS_PREC_FP64: same structure, FP64 sincospi and FP64 accumulation.
+#include <algorithm>
 #include <cstdlib>
 
 #include "viss_kernels.h"
@@ -227,6 +228,7 @@ static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 constexpr int64_t kFwdTilesPerChunk = 256;
 constexpr int64_t kBpRowsPerChunk = 4096;
+constexpr int64_t kMinBlocks = 4096;  // 16 blocks of 4 wavefronts per CU
 
 static int64_t env_int(const char* name, int64_t dflt) {
   const char* v = getenv(name);
@@ -262,10 +264,15 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   const double R = mask ? geom->R_moon : 0.0, orb = mask ? geom->orbit_radius : 0.0;
   const int64_t bx = cdiv(K, 256);
   // 256 tiles x 64 pixels x 32 B = 512 KB per chunk (measured: 191 ms at the
-  // precision workload against 242 ms with 4096-tile chunks)
-  const int64_t tpc = env_int("VISS_FWD_TILES", kFwdTilesPerChunk);
+  // precision workload against 242 ms with 4096-tile chunks); smaller chunks
+  // when few visibility blocks would otherwise leave CUs idle.
+  int64_t tpc = env_int("VISS_FWD_TILES", 0);
+  if (tpc <= 0) {
+    tpc = kFwdTilesPerChunk;
+    while (tpc > 16 && bx * cdiv(sky->ntile, tpc) < kMinBlocks) tpc /= 2;
+  }
+  tpc = std::max(tpc, cdiv(sky->ntile, 65535));  // grid.y limit
   VISS_REQUIRE(bx <= INT32_MAX, "viss_forward: K too large for one launch");
-  VISS_REQUIRE(cdiv(sky->ntile, tpc) <= 65535, "viss_forward: too many pixel chunks");
   dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
 #define VISS_FWD(M, F)                                                                                          \
@@ -313,7 +320,11 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
   const int64_t bx = cdiv(sky->nbtile, 4);
   // 4096 rows x 64 B of records per chunk (measured: 187 ms against 210 ms
   // with 82k-row chunks)
-  int64_t kpc = env_int("VISS_BP_ROWS", kBpRowsPerChunk);
+  int64_t kpc = env_int("VISS_BP_ROWS", 0);
+  if (kpc <= 0) {
+    kpc = kBpRowsPerChunk;
+    while (kpc > 256 && bx * cdiv(K, kpc) < kMinBlocks) kpc /= 2;
+  }
   kpc = cdiv(kpc, 64) * 64;
   if (cdiv(K, kpc) > 65535) kpc = cdiv(cdiv(K, 65535), 64) * 64;
   dim3 grid((unsigned)bx, (unsigned)cdiv(K, kpc));
@@ -392,7 +403,9 @@ int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz
   rc = validate_geom(geom, VISS_MASK);
   if (rc != VISS_OK) return rc;
   const int64_t bx = cdiv(K, 256);
-  const int64_t tpc = kFwdTilesPerChunk;
+  int64_t tpc = kFwdTilesPerChunk;
+  while (tpc > 16 && bx * cdiv(sky->ntile, tpc) < kMinBlocks) tpc /= 2;
+  tpc = std::max(tpc, cdiv(sky->ntile, 65535));
   dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
   k_count_visible<<<grid, 256, 0, as_stream(stream)>>>(sky->pix, sky->cone, sky->tile_valid, sky->ntile, tpc, xyz_a,
                                                        xyz_b, K, geom->R_moon, geom->orbit_radius,
