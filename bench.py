@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 GPU comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the pix2ang / weights HBM measurements")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched persistent-launch comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched one-launch comparison")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the HBM-traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -238,6 +238,40 @@ def main():
                            "rel_l2_adjoint": rel_x, "max_err_forward_rel_to_max": max_v}
         log(f"[bench] fp64 GPU: {w64 * 1e3:.1f} ms/step; mixed/fp64 {value / v64:.2f}x; rel L2 fwd {rel_v:.2e} adj {rel_x:.2e}")
 
+    # snapshot-structured adjoint on the same rows (one phasor per satellite and pixel)
+    pos = torch.from_numpy(inp.positions).to(dev)
+    xs = torch.zeros(sky.P, dtype=torch.float64, device=dev)
+    sky.backproject_snapshots(pos, wl.wavelength, V_mixed, None, mask=True, R_moon=R, out=xs)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(3):
+        xs.zero_()
+        sky.backproject_snapshots(pos, wl.wavelength, V_mixed, None, mask=True, R_moon=R, out=xs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    snap_ms = e0.elapsed_time(e1) / 3
+    out["snapshot_adjoint"] = {
+        "ms": snap_ms, "pairs_per_s": visible / (snap_ms * 1e-3), "speedup_over_per_row_adjoint": adj_avg / snap_ms,
+        "rel_l2_vs_per_row_adjoint": float(torch.linalg.vector_norm(xs - x_mixed) / torch.linalg.vector_norm(x_mixed)),
+        "fwd_plus_snapshot_adjoint_pairs_per_s": 2 * visible / ((fwd_avg + snap_ms) * 1e-3)}
+    log(f"[bench] snapshot adjoint: {snap_ms:.1f} ms ({adj_avg / snap_ms:.2f}x the per-row adjoint)")
+    Vs = torch.zeros_like(V_mixed)
+    sky.forward_snapshots(I, pos, wl.wavelength, mask=True, R_moon=R, out=Vs)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(3):
+        Vs.zero_()
+        sky.forward_snapshots(I, pos, wl.wavelength, mask=True, R_moon=R, out=Vs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sf_ms = e0.elapsed_time(e1) / 3
+    out["snapshot_forward"] = {
+        "ms": sf_ms, "pairs_per_s": visible / (sf_ms * 1e-3), "speedup_over_per_row_forward": fwd_avg / sf_ms,
+        "rel_l2_vs_per_row_forward": float(torch.linalg.vector_norm(Vs - V_mixed) / torch.linalg.vector_norm(V_mixed)),
+        "fwd_plus_adjoint_pairs_per_s": 2 * visible / ((sf_ms + snap_ms) * 1e-3)}
+    log(f"[bench] snapshot forward: {sf_ms:.1f} ms ({fwd_avg / sf_ms:.2f}x the per-row forward)")
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R)
 
@@ -304,8 +338,8 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
 
 
 def batched(ops, geo, dev):
-    """5 frequencies x 64 snapshot groups x 500 snapshots x 28 baselines at nside 128: one persistent
-    (hipGraph-replayed, autotuned) launch against one launch per group."""
+    """5 frequencies x 64 snapshot groups x 500 snapshots x 28 baselines at nside 128: one
+    (hipGraph-replayed, autotuned) work-table launch against one launch per group."""
     import torch
 
     B, uvw, xa, xb, offs, gf = geo.batched_workload()

@@ -140,14 +140,15 @@ int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz
                            const viss_mask_geometry* geom, int64_t* count, void* stream);
 
 /* ---- batched forward: many visibility groups (snapshot groups x
- * frequencies) over one sky in ONE persistent launch.  Replaces the
+ * frequencies) over one sky in ONE launch.  Replaces the
  * reference's per-day loop of forward launches (vissgen3d_noblockage.cu:
  * 270-440) for in-memory batches.  Rows of group g are rows
  * [row_offsets[g], row_offsets[g+1]) of the SoA 3 x K_total arrays uvw /
  * xyz_a / xyz_b (device; they must stay valid for the batch's lifetime);
  * row_offsets is a HOST array of n_groups + 1 entries starting at 0.  Each
- * group gets a compacted list of the sky tiles its satellites can see
- * (uneven lists are balanced by an atomic work queue).  flags: VISS_MASK,
+ * group gets a compacted list of the sky tiles its satellites can see;
+ * the lists are cut into work items (64 rows x a tile range sized per
+ * bucket of list length and group rows) and one wavefront runs each item.  flags: VISS_MASK,
  * VISS_PHASE_TRACK, VISS_PREC_FP64.  The batch keeps a pointer to `sky`. */
 typedef struct viss_batch viss_batch;
 
@@ -156,8 +157,8 @@ int viss_batch_create(viss_sky* sky, int64_t n_groups, const int64_t* row_offset
                       viss_batch** batch, void* stream);
 int viss_batch_destroy(viss_batch* batch);
 /* V[k] += forward for every row k of every group (complex [K_total]).
- * use_graph != 0: the brightness scatter, the queue reset and the persistent
- * kernel are captured once into a hipGraph (re-captured if I, V or the stream
+ * use_graph != 0: the brightness scatter and the batch kernel are
+ * captured once into a hipGraph (re-captured if I, V or the stream
  * change) and replayed; needs a non-default stream. */
 int viss_batch_forward(viss_batch* batch, const double* I, double* V, int use_graph, void* stream);
 /* Times candidate tile-range lengths per (log2 visible tiles, log2 rows)
@@ -166,7 +167,29 @@ int viss_batch_forward(viss_batch* batch, const double* I, double* V, int use_gr
 int viss_batch_autotune(viss_batch* batch, const double* I, double* V_scratch, int reps, void* stream);
 int viss_batch_tuning_save(const viss_batch* batch, const char* path);
 int viss_batch_tuning_load(viss_batch* batch, const char* path, void* stream);
-int viss_batch_stats(const viss_batch* batch, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items);
+/* Sizes of the batch: tiles kept over all groups, tiles before compaction,
+ * work items, blocks of the batch launch (4 items per block).  Every pointer is a host pointer
+ * (nullable). */
+int viss_batch_stats(const viss_batch* batch, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items,
+                     int64_t* grid_blocks);
+
+/* ---- snapshot-structured rows: S snapshots of the same N satellites
+ * (2 <= N <= 16), each snapshot's N(N-1)/2 pairs a<b in row-major
+ * upper-triangle order (a = 0..N-2, b = a+1..N-1), rows snapshot-major;
+ * baseline b = (r_a - r_b) / wavelength.  sat_pos: [S][N][3] FP64 metres.
+ * Same sum as viss_sky_backproject over those rows (the reference's rows are
+ * built this way, vissgen3d_noblockage.cu:302-362), computed with one phasor
+ * per (satellite, pixel) instead of one per (row, pixel).  y, wt: [S*N(N-1)/2]
+ * (complex / FP64, wt nullable).  flags: VISS_MASK, VISS_PHASE_TRACK,
+ * VISS_HERMITIAN; mixed precision only. */
+/* Forward of snapshot-structured rows: V[s*N(N-1)/2 + pair] += sum_p [mask] I_p exp(-2 pi i b.s_p)
+ * (the same sum as viss_sky_forward over those rows); V: complex [S*N(N-1)/2]
+ * ([2x] with VISS_HERMITIAN). */
+int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const double* sat_pos, int64_t S, int32_t N,
+                               double wavelength, const viss_mask_geometry* geom, int flags, double* V, void* stream);
+int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_pos, int64_t S, int32_t N, double wavelength,
+                                   const double* y, const double* wt, const viss_mask_geometry* geom, int flags,
+                                   double* x, void* stream);
 
 /* ---- image utilities after the hot path ----
  * Multi-day aggregation, the notebooks' sum over days

@@ -1,4 +1,4 @@
-"""Batched persistent forward (viss_batch_*): many snapshot groups x frequencies
+"""Batched one-launch forward (viss_batch_*): many snapshot groups x frequencies
 with uneven compacted tile lists in one launch, eager and hipGraph-replayed,
 against per-group launches and the FP64 oracle.
 

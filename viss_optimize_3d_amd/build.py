@@ -24,7 +24,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE, "-I", CSRC,
           "-Wall", "-Wno-unused-function"] + DEVICE_FLAGS
 
-SOURCES = ["viss_sky.hip", "viss_pix2ang.hip", "viss_nudft.hip", "viss_weights.hip", "viss_batch.hip", "viss_image.hip"]
+SOURCES = ["viss_sky.hip", "viss_pix2ang.hip", "viss_nudft.hip", "viss_weights.hip", "viss_batch.hip", "viss_image.hip", "viss_snapshots.hip"]
 HOST_PROGRAMS = {"pix2ang": "host/pix2ang_main.cpp", "vissgen3d": "host/vissgen3d_main.cpp",
                  "vissimage_sum": "host/vissimage_sum_main.cpp"}
 

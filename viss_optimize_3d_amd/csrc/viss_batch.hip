@@ -1,20 +1,21 @@
 // Batched forward over many visibility groups (snapshot groups x frequencies)
-// that share one sky, in ONE persistent launch, replayable as a hipGraph.
+// that share one sky, in ONE launch over a work table, replayable as a hipGraph.
 //
 // The reference runs one day per loop iteration, one launch per kernel, with a
 // device sync after each (vissgen3d_noblockage.cu:270-557).  With occultation
 // each group only sees part of the sky, unevenly: here every group gets a
 // compacted list of the 64-pixel tiles that any of its satellites can see
 // (a conservative cone test of the tile against the bounding cone of the
-// group's satellite directions), and a work table of (group, 256 rows, range
-// of its tile list) items is drained by a persistent grid through an atomic
-// counter.  The per-item tile-range length is chosen per
-// (visible-tile-count, row-count) bucket by an autotuner that times the
+// group's satellite directions), and a work table of (group, 64 rows, range
+// of its tile list) items is covered by one launch, one wavefront per item.
+// The per-item tile-range length is chosen per (visible-tile-count,
+// row-count) bucket by an autotuner that times the
 // candidates on the device (viss_batch_autotune), and the table can be saved
 // and reloaded.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <new>
 #include <vector>
@@ -133,31 +134,32 @@ __global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ 
   if (threadIdx.x == 0) count[g] = running;
 }
 
-__global__ void k_reset_counter(unsigned int* c) { *c = 0u; }
-
+// One launch over the whole work table, one wavefront per item (up to 64
+// rows, one per lane).  The hardware dispatcher is the work queue.  A
+// persistent loop pulling items through an atomic counter measured 2.6x
+// slower: there the FP64 atomics into V that end one item precede the loads
+// of the next, and the compiler emitted per-lane vector loads for the sky
+// data instead of scalar loads (SGPR operands).
 template <bool MASK, bool F64>
 __global__ __launch_bounds__(256) void k_forward_batch(const SkyView sv, const int32_t* __restrict__ tiles,
-                                                       int64_t ntile,
-                                                       const WorkItem* __restrict__ items, int64_t n_items,
-                                                       unsigned int* __restrict__ counter,
-                                                       const double* __restrict__ uvw,
+                                                       int64_t ntile, const WorkItem* __restrict__ items,
+                                                       int64_t n_items, const double* __restrict__ uvw,
                                                        const double* __restrict__ xyza,
                                                        const double* __restrict__ xyzb, int64_t Ktot, double R_moon,
                                                        double orbit_r, int track, double* __restrict__ V) {
-  __shared__ int64_t s_item;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int64_t it = s_item;
-    __syncthreads();
-    if (it >= n_items) break;  // block-uniform: every wave leaves once the table is drained
-    const WorkItem w = items[it];
-    const int64_t k = w.row0 + threadIdx.x;
-    const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, (int)threadIdx.x < w.nrows, R_moon, orbit_r, track);
-    double Sr = 0.0, Si = 0.0;
-    forward_tiles<MASK, F64>(sv, tiles + (int64_t)w.g * ntile, w.j0, w.j1, r, Sr, Si);
-    if (r.act) forward_store(V, k, Ktot, 0, Sr, Si);
-  }
+  const int lane = threadIdx.x & 63;
+  // threadIdx.x >> 6 is the same on every lane of a wavefront, but the
+  // compiler cannot prove it; readfirstlane makes the item (and so every
+  // sky-data address derived from it) uniform, which keeps the tile loop's
+  // loads scalar.
+  const int64_t it = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (it >= n_items) return;  // wave-uniform
+  const WorkItem w = items[it];
+  const int64_t k = w.row0 + lane;
+  const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, lane < w.nrows, R_moon, orbit_r, track);
+  double Sr = 0.0, Si = 0.0;
+  forward_tiles<MASK, F64>(sv, tiles + (int64_t)w.g * ntile, w.j0, w.j1, r, Sr, Si);
+  if (r.act) forward_store(V, k, Ktot, 0, Sr, Si);
 }
 
 }  // namespace viss
@@ -181,8 +183,6 @@ struct viss_batch {
   std::map<int, int> tune;  // bucket -> tiles per item
   WorkItem* items = nullptr;
   int64_t n_items = 0;
-  unsigned int* counter = nullptr;
-  int grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const double* g_I = nullptr;
@@ -218,12 +218,12 @@ static void build_items(const viss_batch* b, const std::vector<int>& groups, std
       auto it = tune.find(bucket_of(nt, rows));
       tpc = it != tune.end() ? it->second : default_tpc(nt);
     }
-    for (int64_t r0 = 0; r0 < rows; r0 += 256)
+    for (int64_t r0 = 0; r0 < rows; r0 += 64)
       for (int64_t j = 0; j < nt; j += tpc) {
         WorkItem w;
         w.row0 = b->offs[g] + r0;
         w.g = g;
-        w.nrows = (int32_t)std::min<int64_t>(256, rows - r0);
+        w.nrows = (int32_t)std::min<int64_t>(64, rows - r0);
         w.j0 = (int32_t)j;
         w.j1 = (int32_t)std::min<int64_t>(nt, j + tpc);
         w.pad0 = w.pad1 = 0;
@@ -231,8 +231,9 @@ static void build_items(const viss_batch* b, const std::vector<int>& groups, std
       }
   }
   // Items that run at the same time should read the same tiles: order by the
-  // position in the (tile-sorted) lists, then by group, so the queue sweeps the
-  // sky once and concurrent items share L2.
+  // position in the (tile-sorted) lists, then by group: blocks are dispatched
+  // in index order, so the launch sweeps the sky once and items resident at
+  // the same time share L2.
   std::stable_sort(out.begin(), out.end(), [](const WorkItem& a, const WorkItem& c) {
     if (a.j0 != c.j0) return a.j0 < c.j0;
     return a.g < c.g;
@@ -242,6 +243,7 @@ static void build_items(const viss_batch* b, const std::vector<int>& groups, std
 static int upload_items(viss_batch* b, const std::vector<WorkItem>& v, hipStream_t s) {
   if (b->items) VISS_HIP(hipFree(b->items));
   b->items = nullptr;
+  VISS_REQUIRE(v.size() < (size_t)INT32_MAX / 2, "viss_batch: work table too large (%zu items)", v.size());
   b->n_items = (int64_t)v.size();
   if (!v.empty()) {
     VISS_HIP(hipMalloc((void**)&b->items, v.size() * sizeof(WorkItem)));
@@ -258,15 +260,15 @@ static int upload_items(viss_batch* b, const std::vector<WorkItem>& v, hipStream
 
 static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, double* V, hipStream_t s) {
   viss_sky* sky = b->sky;
-  k_reset_counter<<<1, 1, 0, s>>>(b->counter);
-  VISS_LAUNCHED(k_reset_counter);
   if (n_items == 0) return VISS_OK;
-  const unsigned grid = (unsigned)std::min<int64_t>(b->grid, n_items);
+  const int64_t blocks = (n_items + 3) / 4;
+  VISS_REQUIRE(blocks <= INT32_MAX, "viss_batch: work table too large");
+  const unsigned grid = (unsigned)blocks;
   const int track = (b->flags & VISS_PHASE_TRACK) ? 1 : 0;
   const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
-#define VISS_FB(M, F)                                                                                            \
-  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky_view(sky), b->tiles, sky->ntile, items, n_items, \
-                                             b->counter, b->uvw, b->xa, b->xb, b->Ktot, b->R, b->orb, track, V)
+#define VISS_FB(M, F)                                                                                             \
+  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky_view(sky), b->tiles, sky->ntile, items, n_items, b->uvw, b->xa, \
+                                             b->xb, b->Ktot, b->R, b->orb, track, V)
   if (mask) {
     if (f64) VISS_FB(true, true); else VISS_FB(true, false);
   } else {
@@ -286,7 +288,6 @@ static void batch_free(viss_batch* b) {
   if (b->tiles) (void)hipFree(b->tiles);
   if (b->d_count) (void)hipFree(b->d_count);
   if (b->items) (void)hipFree(b->items);
-  if (b->counter) (void)hipFree(b->counter);
 }
 
 static int batch_build(viss_batch* b, hipStream_t s) {
@@ -297,7 +298,6 @@ static int batch_build(viss_batch* b, hipStream_t s) {
   VISS_HIP(hipMalloc((void**)&b->gcut, sizeof(float) * G));
   VISS_HIP(hipMalloc((void**)&b->tiles, sizeof(int32_t) * G * sky->ntile));
   VISS_HIP(hipMalloc((void**)&b->d_count, sizeof(int32_t) * G));
-  VISS_HIP(hipMalloc((void**)&b->counter, sizeof(unsigned int)));
   VISS_HIP(hipMemcpyAsync(b->d_offs, b->offs.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, s));
   b->ntiles_g.assign(G, (int32_t)sky->ntile);
   std::vector<int32_t> all;
@@ -315,14 +315,6 @@ static int batch_build(viss_batch* b, hipStream_t s) {
                               hipMemcpyHostToDevice, s));
   }
   VISS_HIP(hipStreamSynchronize(s));
-  int dev = 0, ncu = 0, per_cu = 0;
-  VISS_HIP(hipGetDevice(&dev));
-  VISS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
-  const void* kern = mask ? (f64 ? (const void*)k_forward_batch<true, true> : (const void*)k_forward_batch<true, false>)
-                          : (f64 ? (const void*)k_forward_batch<false, true> : (const void*)k_forward_batch<false, false>);
-  VISS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
-  b->grid = std::max(1, ncu * std::max(1, per_cu));
   std::vector<int> groups(G);
   for (int64_t g = 0; g < G; ++g) groups[g] = (int)g;
   std::vector<WorkItem> v;
@@ -515,13 +507,15 @@ int viss_batch_tuning_load(viss_batch* b, const char* path, void* stream) {
   return upload_items(b, v, as_stream(stream));
 }
 
-int viss_batch_stats(const viss_batch* b, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items) {
+int viss_batch_stats(const viss_batch* b, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items,
+                     int64_t* grid_blocks) {
   VISS_REQUIRE(b != nullptr, "viss_batch_stats: NULL batch");
   int64_t vt = 0;
   for (int32_t c : b->ntiles_g) vt += c;
   if (visible_tiles) *visible_tiles = vt;
   if (total_tiles) *total_tiles = b->G * b->sky->ntile;
   if (n_items) *n_items = b->n_items;
+  if (grid_blocks) *grid_blocks = (b->n_items + 3) / 4;
   return VISS_OK;
 }
 

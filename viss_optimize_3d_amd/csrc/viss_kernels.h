@@ -1,5 +1,5 @@
 // Device-side building blocks shared by the forward / back-projection kernels
-// (viss_nudft.hip) and the batched persistent forward (viss_batch.hip).
+// (viss_nudft.hip) and the batched work-table forward (viss_batch.hip).
 #pragma once
 
 #include "viss_internal.h"

@@ -1,0 +1,411 @@
+// Snapshot-structured back-projection (adjoint): rows grouped as S snapshots
+// of the same N satellites, every pair a<b of a snapshot one row, with
+// baseline b = (r_a - r_b) / lambda.  This is the layout the reference's data
+// come in (uvw rows built from satellite pairs, vissgen3d_noblockage.cu:
+// 302-362) and the layout of every workload here.
+//
+// With G_i(p) = [s_p . r_i >= t_i] exp(+2 pi i d_i(p)), d_i = r_i . s_p / lambda,
+// one row's term is  y_ab G_a conj(G_b)  = y_ab m_kp exp(+2 pi i b . s_p), so
+//   x_p += Re sum_s sum_a G_a(p) H_a(p),   H_a = sum_{b>a} y_ab conj(G_b(p)).
+// Per pixel and snapshot the N satellite phasors cost N FP64 dot products and
+// N hardware sin/cos pairs instead of N(N-1)/2; each row is then one complex
+// multiply-add.  The sum over (row, pixel) pairs is the same as the per-row
+// kernel's (k_backproject): the occultation factor is exact per satellite
+// (FP64 s.r >= t, the per-row test's two factors), the phase d_a - d_b equals
+// b . s up to FP64 rounding, and the reductions are the same blocked FP32 ->
+// FP64 sums.
+#include <algorithm>
+
+#include "viss_kernels.h"
+
+namespace viss {
+
+struct SatRec {  // one satellite of one snapshot
+  double x, y, z, t;
+};
+static_assert(sizeof(SatRec) == 32, "SatRec is 32 B");
+
+struct SatCls {  // unit direction and unit cut, for tile skipping
+  float ax, ay, az, c;
+};
+
+// Per (snapshot, satellite): exact occultation record and classification record.
+__global__ void k_snap_sats(const double* __restrict__ pos, int64_t SN, double R_moon, double orbit_r, int mask,
+                            SatRec* __restrict__ rec, SatCls* __restrict__ cls) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < SN; i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    SatRec r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    r.t = mask ? mask_threshold(x, y, z, R_moon, orbit_r) : -1e300;
+    rec[i] = r;
+    const double nr = sqrt(x * x + y * y + z * z);
+    const double inv = nr > 0 ? 1.0 / nr : 0.0;
+    SatCls c;
+    c.ax = (float)(x * inv);
+    c.ay = (float)(y * inv);
+    c.az = (float)(z * inv);
+    c.c = (mask && nr > 0) ? (float)(r.t * inv) : -2.0f;
+    cls[i] = c;
+  }
+}
+
+// Per row: y' = wt * y [* exp(-2 pi i w)], w = (z_a - z_b) / lambda, as FP32 complex.
+__global__ void k_snap_rows(const double* __restrict__ pos, const double* __restrict__ y,
+                            const double* __restrict__ wt, int64_t S, int N, double inv_lambda, int track,
+                            float2* __restrict__ yw) {
+  const int NB = N * (N - 1) / 2;
+  const int64_t K = S * NB;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
+    double yr = y[2 * k], yi = y[2 * k + 1];
+    if (track) {
+      const int64_t s = k / NB;
+      int q = (int)(k - s * NB), a = 0;
+      while (q >= N - 1 - a) {
+        q -= N - 1 - a;
+        ++a;
+      }
+      const int b = a + 1 + q;
+      const double w = (pos[3 * (s * N + a) + 2] - pos[3 * (s * N + b) + 2]) * inv_lambda;
+      double sn, cs;
+      sincospi(-2.0 * (w - rint(w)), &sn, &cs);
+      const double nr = yr * cs - yi * sn, ni = yr * sn + yi * cs;
+      yr = nr;
+      yi = ni;
+    }
+    if (wt) {
+      yr *= wt[k];
+      yi *= wt[k];
+    }
+    yw[k] = make_float2((float)yr, (float)yi);
+  }
+}
+
+// Grid: x = ceil(nbtile / 4) blocks of 4 wavefronts (one kBpTile-pixel tile per
+// wavefront), y = snapshot chunks.  NMAX >= N bounds the register arrays.
+template <bool MASK, int NMAX>
+__global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restrict__ pix,
+                                                          const float4* __restrict__ bcone, int64_t nbtile,
+                                                          const SatRec* __restrict__ rec,
+                                                          const SatCls* __restrict__ cls,
+                                                          const float2* __restrict__ yw, int64_t S, int N,
+                                                          double inv_lambda, int64_t s_per_chunk, double herm,
+                                                          double* __restrict__ x) {
+  constexpr int PPL = kBpPixPerLane;
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= nbtile) return;  // wave-uniform
+  double l[PPL], m[PPL], n[PPL];
+  int32_t idx[PPL];
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    const PixPack q = pix[tile * kBpTile + i * 64 + lane];
+    l[i] = q.l;
+    m[i] = q.m;
+    n[i] = q.n;
+    idx[i] = q.idx;
+  }
+  const float4 cn = bcone[tile];
+  const int NB = N * (N - 1) / 2;
+  const int64_t s0 = (int64_t)blockIdx.y * s_per_chunk;
+  const int64_t s1 = min(S, s0 + s_per_chunk);
+  double acc[PPL];
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) acc[i] = 0.0;
+  for (int64_t s = s0; s < s1; ++s) {
+    const SatRec* __restrict__ R = rec + s * N;
+    if (MASK) {
+      // skip the snapshot when no satellite can see any pixel of this tile
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        if (i < N) {
+          const SatCls c = cls[s * N + i];
+          const float d = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+          any |= d + cn.w >= c.c - kClsEps;
+        }
+      }
+      if (__ballot(any) == 0) continue;  // every lane agrees: tile and snapshot are wave-uniform
+    }
+    float gr[NMAX][PPL], gi[NMAX][PPL];
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) {
+      if (i < N) {
+        const SatRec q = R[i];
+#pragma unroll
+        for (int p = 0; p < PPL; ++p) {
+          const double D = fma(q.x, l[p], fma(q.y, m[p], q.z * n[p]));
+          float sn, cs;
+          sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
+          const bool vis = !MASK || (D >= q.t);
+          gr[i][p] = vis ? cs : 0.f;
+          gi[i][p] = vis ? sn : 0.f;
+        }
+      }
+    }
+    const float2* __restrict__ Y = yw + s * NB;
+    float part[PPL];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) part[p] = 0.f;
+    int pr = 0;
+#pragma unroll
+    for (int a = 0; a < NMAX - 1; ++a) {
+      if (a < N - 1) {
+        float hr[PPL], hi[PPL];
+#pragma unroll
+        for (int p = 0; p < PPL; ++p) hr[p] = hi[p] = 0.f;
+#pragma unroll
+        for (int b = a + 1; b < NMAX; ++b) {
+          if (b < N) {
+            const float2 yv = Y[pr++];
+#pragma unroll
+            for (int p = 0; p < PPL; ++p) {  // H += y conj(G_b)
+              hr[p] = fmaf(yv.x, gr[b][p], fmaf(yv.y, gi[b][p], hr[p]));
+              hi[p] = fmaf(yv.y, gr[b][p], fmaf(-yv.x, gi[b][p], hi[p]));
+            }
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < PPL; ++p) part[p] = fmaf(gr[a][p], hr[p], fmaf(-gi[a][p], hi[p], part[p]));
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) acc[p] += (double)part[p];
+  }
+#pragma unroll
+  for (int p = 0; p < PPL; ++p)
+    if (idx[p] >= 0) unsafeAtomicAdd(&x[idx[p]], herm * acc[p]);
+}
+
+
+// Forward over snapshot-structured rows: V_ab(s) = sum_p I_p conj(G_a(p)) G_b(p)
+// (= sum_p I_p m_kp exp(-2 pi i b . s_p)).  One wavefront per (snapshot,
+// pixel range); each lane owns one pixel of every 64-pixel tile of the range
+// and keeps FP32 partial sums for every pair of the snapshot in registers;
+// at the end of the range the partials are summed across the wavefront
+// (butterfly) and added into V with FP64 atomics.
+template <bool MASK, int NMAX>
+__global__ __launch_bounds__(256) void k_forward_snap(const PixPack* __restrict__ pix, const float4* __restrict__ cone,
+                                                      int64_t ntile, int64_t tiles_per_range,
+                                                      const SatRec* __restrict__ rec, const SatCls* __restrict__ cls,
+                                                      int64_t S, int N, double inv_lambda, int track, int herm,
+                                                      double* __restrict__ V) {
+  constexpr int NBM = NMAX * (NMAX - 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= S) return;  // wave-uniform
+  const int64_t t0 = (int64_t)blockIdx.y * tiles_per_range;
+  const int64_t t1 = min(ntile, t0 + tiles_per_range);
+  const SatRec* __restrict__ R = rec + s * N;
+  float ar[NBM], ai[NBM];
+#pragma unroll
+  for (int q = 0; q < NBM; ++q) ar[q] = ai[q] = 0.f;
+  for (int64_t t = t0; t < t1; ++t) {
+    if (MASK) {
+      const float4 cn = cone[t];
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        if (i < N) {
+          const SatCls c = cls[s * N + i];
+          const float d = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+          any |= d + cn.w >= c.c - kClsEps;
+        }
+      }
+      if (__ballot(any) == 0) continue;  // no satellite sees this tile
+    }
+    const PixPack q = pix[t * kTile + lane];
+    float gr[NMAX], gi[NMAX];
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) {
+      gr[i] = gi[i] = 0.f;
+      if (i < N) {
+        const SatRec r = R[i];
+        const double D = fma(r.x, q.l, fma(r.y, q.m, r.z * q.n));
+        float sn, cs;
+        sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
+        const bool vis = !MASK || (D >= r.t);
+        gr[i] = vis ? cs : 0.f;
+        gi[i] = vis ? sn : 0.f;
+      }
+    }
+    int pr = 0;
+#pragma unroll
+    for (int a = 0; a < NMAX - 1; ++a) {
+      const float fr = q.I * gr[a], fi = -q.I * gi[a];  // I conj(G_a)
+#pragma unroll
+      for (int b = a + 1; b < NMAX; ++b, ++pr) {
+        if (b < N) {  // F_a G_b
+          ar[pr] = fmaf(fr, gr[b], fmaf(-fi, gi[b], ar[pr]));
+          ai[pr] = fmaf(fr, gi[b], fmaf(fi, gr[b], ai[pr]));
+        }
+      }
+    }
+  }
+  // sum over the wavefront's pixels
+#pragma unroll
+  for (int q = 0; q < NBM; ++q) {
+    float vr = ar[q], vi = ai[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      vr += __shfl_xor(vr, o, 64);
+      vi += __shfl_xor(vi, o, 64);
+    }
+    ar[q] = vr;
+    ai[q] = vi;
+  }
+  if (lane == 0) {
+    const int NB = N * (N - 1) / 2;
+    const int64_t K = S * NB;
+    int pm = 0;
+#pragma unroll
+    for (int a = 0; a < NMAX - 1; ++a) {
+#pragma unroll
+      for (int b = a + 1; b < NMAX; ++b, ++pm) {
+        if (b < N) {
+          const int64_t k = s * NB + (a * N - a * (a + 1) / 2 + (b - a - 1));
+          double vr = ar[pm], vi = ai[pm];
+          if (track) {  // V e^{+2 pi i w}, w = (z_a - z_b) / lambda
+            const double w = (R[a].z - R[b].z) * inv_lambda;
+            double sn, cs;
+            sincospi(2.0 * (w - rint(w)), &sn, &cs);
+            const double nr = vr * cs - vi * sn, ni = vr * sn + vi * cs;
+            vr = nr;
+            vi = ni;
+          }
+          unsafeAtomicAdd(&V[2 * k], vr);
+          unsafeAtomicAdd(&V[2 * k + 1], vi);
+          if (herm) {
+            unsafeAtomicAdd(&V[2 * (k + K)], vr);
+            unsafeAtomicAdd(&V[2 * (k + K) + 1], -vi);
+          }
+        }
+      }
+    }
+  }
+}
+
+static int64_t cdiv_s(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+static unsigned grid_rows(int64_t n) {
+  int64_t g = cdiv_s(n, 256);
+  if (g > 65536) g = 65536;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace viss
+
+using namespace viss;
+
+extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_pos, int64_t S, int32_t N,
+                                              double wavelength, const double* y, const double* wt,
+                                              const viss_mask_geometry* geom, int flags, double* x, void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_backproject_snapshots: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE(S >= 0 && N >= 2 && N <= 16, "viss_sky_backproject_snapshots: need S >= 0 and 2 <= N <= 16");
+  VISS_REQUIRE(wavelength > 0.0, "viss_sky_backproject_snapshots: wavelength must be > 0");
+  VISS_REQUIRE((flags & ~(VISS_MASK | VISS_PHASE_TRACK | VISS_HERMITIAN)) == 0,
+               "viss_sky_backproject_snapshots: unsupported flags 0x%x (mixed precision only)", flags);
+  if (S == 0) return VISS_OK;
+  VISS_REQUIRE(sat_pos && y && x, "viss_sky_backproject_snapshots: NULL argument");
+  const bool mask = flags & VISS_MASK;
+  if (mask) {
+    VISS_REQUIRE(geom != nullptr && geom->R_moon >= 0.0 &&
+                     (geom->orbit_radius <= 0.0 || geom->orbit_radius > geom->R_moon),
+                 "viss_sky_backproject_snapshots: bad geometry");
+  }
+  hipStream_t s = as_stream(stream);
+  const int64_t NB = (int64_t)N * (N - 1) / 2, SN = S * N, K = S * NB;
+  const size_t b_rec = sizeof(SatRec) * SN, b_cls = sizeof(SatCls) * SN, b_y = sizeof(float2) * K;
+  rc = sky->ws.reserve(b_rec + b_cls + b_y + 512, s);
+  if (rc != VISS_OK) return rc;
+  char* base = (char*)sky->ws.p;
+  SatRec* rec = (SatRec*)base;
+  SatCls* cls = (SatCls*)(base + b_rec);
+  float2* yw = (float2*)(base + ((b_rec + b_cls + 255) / 256) * 256);
+  const double inv_lambda = 1.0 / wavelength;
+  k_snap_sats<<<grid_rows(SN), 256, 0, s>>>(sat_pos, SN, mask ? geom->R_moon : 0.0, mask ? geom->orbit_radius : 0.0,
+                                            mask ? 1 : 0, rec, cls);
+  VISS_LAUNCHED(k_snap_sats);
+  k_snap_rows<<<grid_rows(K), 256, 0, s>>>(sat_pos, y, wt, S, N, inv_lambda, (flags & VISS_PHASE_TRACK) ? 1 : 0, yw);
+  VISS_LAUNCHED(k_snap_rows);
+  const int64_t bx = cdiv_s(sky->nbtile, 4);
+  // about 4096 rows per chunk, as the per-row kernel; more chunks for small launches
+  int64_t spc = std::max<int64_t>(1, 4096 / NB);
+  while (spc > 1 && bx * cdiv_s(S, spc) < 4096) spc = (spc + 1) / 2;
+  if (cdiv_s(S, spc) > 65535) spc = cdiv_s(S, 65535);
+  dim3 grid((unsigned)bx, (unsigned)cdiv_s(S, spc));
+  const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
+#define VISS_BPS(M, NM)                                                                                             \
+  k_backproject_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, rec, cls, yw, S, N, inv_lambda, \
+                                                 spc, herm, x)
+  if (N <= 8) {
+    if (mask) VISS_BPS(true, 8); else VISS_BPS(false, 8);
+  } else {
+    if (mask) VISS_BPS(true, 16); else VISS_BPS(false, 16);
+  }
+#undef VISS_BPS
+  VISS_LAUNCHED(k_backproject_snap);
+  return VISS_OK;
+}
+
+extern "C" int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const double* sat_pos, int64_t S, int32_t N,
+                                          double wavelength, const viss_mask_geometry* geom, int flags, double* V,
+                                          void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_forward_snapshots: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE(S >= 0 && N >= 2 && N <= 16, "viss_sky_forward_snapshots: need S >= 0 and 2 <= N <= 16");
+  VISS_REQUIRE(wavelength > 0.0, "viss_sky_forward_snapshots: wavelength must be > 0");
+  VISS_REQUIRE((flags & ~(VISS_MASK | VISS_PHASE_TRACK | VISS_HERMITIAN)) == 0,
+               "viss_sky_forward_snapshots: unsupported flags 0x%x (mixed precision only)", flags);
+  if (S == 0) return VISS_OK;
+  VISS_REQUIRE(I && sat_pos && V, "viss_sky_forward_snapshots: NULL argument");
+  const bool mask = flags & VISS_MASK;
+  if (mask) {
+    VISS_REQUIRE(geom != nullptr && geom->R_moon >= 0.0 &&
+                     (geom->orbit_radius <= 0.0 || geom->orbit_radius > geom->R_moon),
+                 "viss_sky_forward_snapshots: bad geometry");
+  }
+  hipStream_t s = as_stream(stream);
+  rc = sky_set_brightness(sky, I, false, s);
+  if (rc != VISS_OK) return rc;
+  const int64_t SN = S * N;
+  const size_t b_rec = sizeof(SatRec) * SN, b_cls = sizeof(SatCls) * SN;
+  rc = sky->ws.reserve(b_rec + b_cls + 256, s);
+  if (rc != VISS_OK) return rc;
+  SatRec* rec = (SatRec*)sky->ws.p;
+  SatCls* cls = (SatCls*)((char*)sky->ws.p + b_rec);
+  const double inv_lambda = 1.0 / wavelength;
+  k_snap_sats<<<grid_rows(SN), 256, 0, s>>>(sat_pos, SN, mask ? geom->R_moon : 0.0, mask ? geom->orbit_radius : 0.0,
+                                            mask ? 1 : 0, rec, cls);
+  VISS_LAUNCHED(k_snap_sats);
+  const int64_t bx = cdiv_s(S, 4);
+  // 256 tiles (16,384 pixels, 512 KB of pixel data) per range, fewer when the
+  // snapshots alone would not fill the device
+  int64_t tpr = 256;
+  while (tpr > 16 && bx * cdiv_s(sky->ntile, tpr) < 4096) tpr /= 2;
+  tpr = std::max(tpr, cdiv_s(sky->ntile, 65535));
+  VISS_REQUIRE(bx <= INT32_MAX, "viss_sky_forward_snapshots: too many snapshots");
+  dim3 grid((unsigned)bx, (unsigned)cdiv_s(sky->ntile, tpr));
+  const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
+#define VISS_FS(M, NM)                                                                                             \
+  k_forward_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->cone, sky->ntile, tpr, rec, cls, S, N, inv_lambda, track, \
+                                             herm, V)
+  if (N <= 4) {
+    if (mask) VISS_FS(true, 4); else VISS_FS(false, 4);
+  } else if (N <= 8) {
+    if (mask) VISS_FS(true, 8); else VISS_FS(false, 8);
+  } else if (N <= 10) {
+    if (mask) VISS_FS(true, 10); else VISS_FS(false, 10);
+  } else if (N <= 12) {
+    if (mask) VISS_FS(true, 12); else VISS_FS(false, 12);
+  } else {
+    if (mask) VISS_FS(true, 16); else VISS_FS(false, 16);
+  }
+#undef VISS_FS
+  VISS_LAUNCHED(k_forward_snap);
+  return VISS_OK;
+}

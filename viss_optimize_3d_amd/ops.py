@@ -163,6 +163,49 @@ class Sky:
                                                              _stream()))
         return out
 
+    def forward_snapshots(self, I: torch.Tensor, sat_pos: torch.Tensor, wavelength: float, *, mask: bool = False,
+                          R_moon: float = REF_R_MOON, orbit_radius: float = 0.0, phase_track: bool = False,
+                          hermitian: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Forward of snapshot-structured rows (sat_pos [S, N, 3]; rows = pairs a<b per snapshot)."""
+        L = _lib.load()
+        I = _dev(I, torch.float64)
+        pos = _dev(sat_pos, torch.float64)
+        if pos.dim() != 3 or pos.shape[2] != 3:
+            raise ValueError("sat_pos must be [S, N, 3]")
+        S, N = pos.shape[0], pos.shape[1]
+        rows = S * N * (N - 1) // 2 * (2 if hermitian else 1)
+        if out is None:
+            out = torch.zeros(rows, dtype=torch.complex128, device=self.device)
+        elif out.dtype != torch.complex128 or out.numel() != rows or not out.is_contiguous():
+            raise ValueError("out must be a contiguous complex128 tensor of the row count")
+        check("viss_sky_forward_snapshots",
+              L.viss_sky_forward_snapshots(self._h, _p(I), _p(pos), S, N, float(wavelength),
+                                           _geom(R_moon, orbit_radius), _flags(mask, phase_track, False, hermitian),
+                                           _p(out), _stream()))
+        return out
+
+    def backproject_snapshots(self, sat_pos: torch.Tensor, wavelength: float, y: torch.Tensor,
+                              wt: Optional[torch.Tensor] = None, *, mask: bool = False, R_moon: float = REF_R_MOON,
+                              orbit_radius: float = 0.0, phase_track: bool = False, hermitian: bool = False,
+                              out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Back-projection of snapshot-structured rows (sat_pos [S, N, 3]; rows = pairs a<b per snapshot)."""
+        L = _lib.load()
+        pos = _dev(sat_pos, torch.float64)
+        if pos.dim() != 3 or pos.shape[2] != 3:
+            raise ValueError("sat_pos must be [S, N, 3]")
+        S, N = pos.shape[0], pos.shape[1]
+        y = _dev(y, torch.complex128)
+        if y.numel() != S * N * (N - 1) // 2:
+            raise ValueError("y must have S * N(N-1)/2 entries")
+        w = None if wt is None else _dev(wt, torch.float64)
+        if out is None:
+            out = torch.zeros(self.P, dtype=torch.float64, device=self.device)
+        check("viss_sky_backproject_snapshots",
+              L.viss_sky_backproject_snapshots(self._h, _p(pos), S, N, float(wavelength), _p(y), _p(w),
+                                               _geom(R_moon, orbit_radius), _flags(mask, phase_track, False, hermitian),
+                                               _p(out), _stream()))
+        return out
+
     def count_visible(self, xyz_a, xyz_b, *, R_moon: float = REF_R_MOON, orbit_radius: float = 0.0) -> torch.Tensor:
         L = _lib.load()
         xa = _dev(xyz_a, torch.float64)
@@ -202,7 +245,7 @@ def backproject(lmn, uvw, y, wt=None, xyz_a=None, xyz_b=None, **kw):
 
 
 class Batch:
-    """Many visibility groups over one Sky, forward in one persistent launch
+    """Many visibility groups over one Sky, forward in one launch over a work table
     (``viss_batch``).  ``row_offsets``: host sequence of n_groups + 1 row
     offsets into the concatenated [3, K_total] uvw / xyz arrays."""
 
@@ -260,10 +303,10 @@ class Batch:
         check("viss_batch_tuning_load", _lib.load().viss_batch_tuning_load(self._h, path.encode(), _stream()))
 
     def stats(self):
-        vt, tt, ni = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        vt, tt, ni, gb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check("viss_batch_stats", _lib.load().viss_batch_stats(self._h, ctypes.byref(vt), ctypes.byref(tt),
-                                                               ctypes.byref(ni)))
-        return {"visible_tiles": vt.value, "total_tiles": tt.value, "n_items": ni.value}
+                                                               ctypes.byref(ni), ctypes.byref(gb)))
+        return {"visible_tiles": vt.value, "total_tiles": tt.value, "n_items": ni.value, "grid_blocks": gb.value}
 
 
 def image_accumulate(img: torch.Tensor, total: torch.Tensor, nan_to_zero: bool = True) -> torch.Tensor:
