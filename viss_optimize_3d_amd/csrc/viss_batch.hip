@@ -148,11 +148,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const SkyView sv, const i
                                                        const double* __restrict__ xyzb, int64_t Ktot, double R_moon,
                                                        double orbit_r, int track, double* __restrict__ V) {
   const int lane = threadIdx.x & 63;
-  // threadIdx.x >> 6 is the same on every lane of a wavefront, but the
-  // compiler cannot prove it; readfirstlane makes the item (and so every
-  // sky-data address derived from it) uniform, which keeps the tile loop's
-  // loads scalar.
-  const int64_t it = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t it = (int64_t)blockIdx.x * 4 + wave_in_block();
   if (it >= n_items) return;  // wave-uniform
   const WorkItem w = items[it];
   const int64_t k = w.row0 + lane;

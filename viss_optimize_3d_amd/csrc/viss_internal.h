@@ -75,6 +75,13 @@ constexpr int kBpPixPerLane = 2; // back-projection pixels per lane (measured be
 constexpr int kBpTile = 64 * kBpPixPerLane;
 constexpr int kPadTo = kBpTile;  // sky padding granularity
 
+// Index of the calling wavefront in its block.  threadIdx.x >> 6 is the same
+// on every lane of a wavefront, but the compiler cannot prove it;
+// readfirstlane makes it (and every address derived from it) uniform, so
+// per-wavefront data is fetched with scalar loads into SGPRs instead of
+// per-lane vector loads that would each hold a copy in VGPRs.
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // Occultation threshold t(r): s visible from r iff s.r >= t
 // (vissgen3d_noblockage.cu:88-94,240-243 in dot-product form).
 __host__ __device__ inline double mask_threshold(double x, double y, double z, double R_moon, double orbit_radius) {

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
                                                      double* __restrict__ x) {
   constexpr int PPL = kBpPixPerLane;
   const int lane = threadIdx.x & 63;
-  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
   if (tile >= nbtile) return;  // wave-uniform
   double l[PPL], m[PPL], n[PPL];
   int32_t idx[PPL];
@@ -141,26 +141,33 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
       float a[PPL];
 #pragma unroll
       for (int i = 0; i < PPL; ++i) a[i] = 0.f;
-      while (anym) {
-        const int j = __builtin_ctzll(anym);
-        anym &= anym - 1ull;
+      // Rows that see the whole tile, then rows that see part of it: two
+      // loops rather than a branch per row, so the accumulators stay in
+      // place (a per-row branch made the compiler copy them every row).
+      uint64_t fm = anym & fullm;
+      while (fm) {
+        const int j = __builtin_ctzll(fm);
+        fm &= fm - 1ull;
         const VisHot h = hot[kb + j];
-        if (!MASK || ((fullm >> j) & 1ull)) {
 #pragma unroll
-          for (int i = 0; i < PPL; ++i) {
-            float s, c;
-            sincos_rev(rev_fp32(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i]))), s, c);
-            a[i] = fmaf(h.yr, c, fmaf(-h.yi, s, a[i]));
-          }
-        } else {
-          const VisMask q = vm[kb + j];
+        for (int i = 0; i < PPL; ++i) {
+          float s, c;
+          sincos_rev(rev_fp32(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i]))), s, c);
+          a[i] = fmaf(h.yr, c, fmaf(-h.yi, s, a[i]));
+        }
+      }
+      uint64_t pm = MASK ? (anym & ~fullm) : 0ull;
+      while (pm) {
+        const int j = __builtin_ctzll(pm);
+        pm &= pm - 1ull;
+        const VisHot h = hot[kb + j];
+        const VisMask q = vm[kb + j];
 #pragma unroll
-          for (int i = 0; i < PPL; ++i) {
-            float s, c;
-            sincos_rev(rev_fp32(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i]))), s, c);
-            const float t = fmaf(h.yr, c, -h.yi * s);
-            a[i] += both_visible(l[i], m[i], n[i], q) ? t : 0.f;
-          }
+        for (int i = 0; i < PPL; ++i) {
+          float s, c;
+          sincos_rev(rev_fp32(fma(h.u, l[i], fma(h.v, m[i], h.w * n[i]))), s, c);
+          const float t = fmaf(h.yr, c, -h.yi * s);
+          a[i] += both_visible(l[i], m[i], n[i], q) ? t : 0.f;
         }
       }
 #pragma unroll

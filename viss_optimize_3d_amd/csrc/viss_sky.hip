@@ -113,7 +113,7 @@ __device__ inline int wave_isum(int v) {
 __global__ void k_tile_cones(const PixPack* __restrict__ pix, int64_t ntiles, int per_lane, float4* __restrict__ cone,
                              int32_t* __restrict__ valid_out) {
   const int lane = threadIdx.x & 63;
-  const int64_t t = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t t = blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block();
   if (t >= ntiles) return;
   const int64_t base = t * 64 * per_lane;
   double sx = 0, sy = 0, sz = 0;

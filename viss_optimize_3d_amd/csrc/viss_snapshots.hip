@@ -29,17 +29,29 @@ struct SatCls {  // unit direction and unit cut, for tile skipping
   float ax, ay, az, c;
 };
 
-// Per (snapshot, satellite): exact occultation record and classification record.
-__global__ void k_snap_sats(const double* __restrict__ pos, int64_t SN, double R_moon, double orbit_r, int mask,
-                            SatRec* __restrict__ rec, SatCls* __restrict__ cls) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < SN; i += (int64_t)gridDim.x * blockDim.x) {
+// Per (snapshot, satellite): exact occultation record and classification
+// record, NP >= N records per snapshot.  Records N..NP-1 of a snapshot are
+// padding: a satellite at the origin that sees nothing (its phasor is 0 under
+// the mask, and rows that involve it are never written).
+__global__ void k_snap_sats(const double* __restrict__ pos, int64_t S, int N, int NP, double R_moon, double orbit_r,
+                            int mask, SatRec* __restrict__ rec, SatCls* __restrict__ cls) {
+  const int64_t SNP = S * NP;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < SNP; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t sn = j / NP;
+    const int k = (int)(j - sn * NP);
+    if (k >= N) {
+      rec[j] = SatRec{0.0, 0.0, 0.0, 1e300};
+      cls[j] = SatCls{0.f, 0.f, 1.f, 4.0f};
+      continue;
+    }
+    const int64_t i = sn * N + k;
     const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
     SatRec r;
     r.x = x;
     r.y = y;
     r.z = z;
     r.t = mask ? mask_threshold(x, y, z, R_moon, orbit_r) : -1e300;
-    rec[i] = r;
+    rec[j] = r;
     const double nr = sqrt(x * x + y * y + z * z);
     const double inv = nr > 0 ? 1.0 / nr : 0.0;
     SatCls c;
@@ -47,7 +59,7 @@ __global__ void k_snap_sats(const double* __restrict__ pos, int64_t SN, double R
     c.ay = (float)(y * inv);
     c.az = (float)(z * inv);
     c.c = (mask && nr > 0) ? (float)(r.t * inv) : -2.0f;
-    cls[i] = c;
+    cls[j] = c;
   }
 }
 
@@ -94,7 +106,7 @@ __global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restr
                                                           double* __restrict__ x) {
   constexpr int PPL = kBpPixPerLane;
   const int lane = threadIdx.x & 63;
-  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
   if (tile >= nbtile) return;  // wave-uniform
   double l[PPL], m[PPL], n[PPL];
   int32_t idx[PPL];
@@ -186,18 +198,18 @@ __global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restr
 // at the end of the range the partials are summed across the wavefront
 // (butterfly) and added into V with FP64 atomics.
 template <bool MASK, int NMAX>
-__global__ __launch_bounds__(256) void k_forward_snap(const PixPack* __restrict__ pix, const float4* __restrict__ cone,
+__global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(const PixPack* __restrict__ pix, const float4* __restrict__ cone,
                                                       int64_t ntile, int64_t tiles_per_range,
                                                       const SatRec* __restrict__ rec, const SatCls* __restrict__ cls,
                                                       int64_t S, int N, double inv_lambda, int track, int herm,
                                                       double* __restrict__ V) {
   constexpr int NBM = NMAX * (NMAX - 1) / 2;
   const int lane = threadIdx.x & 63;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t s = (int64_t)blockIdx.x * 4 + wave_in_block();
   if (s >= S) return;  // wave-uniform
   const int64_t t0 = (int64_t)blockIdx.y * tiles_per_range;
   const int64_t t1 = min(ntile, t0 + tiles_per_range);
-  const SatRec* __restrict__ R = rec + s * N;
+  const SatRec* __restrict__ R = rec + s * NMAX;
   float ar[NBM], ai[NBM];
 #pragma unroll
   for (int q = 0; q < NBM; ++q) ar[q] = ai[q] = 0.f;
@@ -206,44 +218,45 @@ __global__ __launch_bounds__(256) void k_forward_snap(const PixPack* __restrict_
       const float4 cn = cone[t];
       bool any = false;
 #pragma unroll
-      for (int i = 0; i < NMAX; ++i) {
-        if (i < N) {
-          const SatCls c = cls[s * N + i];
-          const float d = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
-          any |= d + cn.w >= c.c - kClsEps;
-        }
+      for (int i = 0; i < NMAX; ++i) {  // padding records never see a tile
+        const SatCls c = cls[s * NMAX + i];
+        const float d = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+        any |= d + cn.w >= c.c - kClsEps;
       }
       if (__ballot(any) == 0) continue;  // no satellite sees this tile
     }
     const PixPack q = pix[t * kTile + lane];
+    // Every record up to NMAX is evaluated (no per-satellite branches in the
+    // loop); pairs with a padding satellite are accumulated and dropped.
     float gr[NMAX], gi[NMAX];
 #pragma unroll
     for (int i = 0; i < NMAX; ++i) {
-      gr[i] = gi[i] = 0.f;
-      if (i < N) {
-        const SatRec r = R[i];
-        const double D = fma(r.x, q.l, fma(r.y, q.m, r.z * q.n));
-        float sn, cs;
-        sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
-        const bool vis = !MASK || (D >= r.t);
-        gr[i] = vis ? cs : 0.f;
-        gi[i] = vis ? sn : 0.f;
-      }
+      const SatRec r = R[i];
+      const double D = fma(r.x, q.l, fma(r.y, q.m, r.z * q.n));
+      float sn, cs;
+      sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
+      const bool vis = !MASK || (D >= r.t);
+      gr[i] = vis ? cs : 0.f;
+      gi[i] = vis ? sn : 0.f;
     }
     int pr = 0;
 #pragma unroll
     for (int a = 0; a < NMAX - 1; ++a) {
       const float fr = q.I * gr[a], fi = -q.I * gi[a];  // I conj(G_a)
 #pragma unroll
-      for (int b = a + 1; b < NMAX; ++b, ++pr) {
-        if (b < N) {  // F_a G_b
-          ar[pr] = fmaf(fr, gr[b], fmaf(-fi, gi[b], ar[pr]));
-          ai[pr] = fmaf(fr, gi[b], fmaf(fi, gr[b], ai[pr]));
-        }
+      for (int b = a + 1; b < NMAX; ++b, ++pr) {  // F_a G_b
+        ar[pr] = fmaf(fr, gr[b], fmaf(-fi, gi[b], ar[pr]));
+        ai[pr] = fmaf(fr, gi[b], fmaf(fi, gr[b], ai[pr]));
       }
     }
   }
-  // sum over the wavefront's pixels
+  // Sum over the wavefront's pixels (xor butterfly: every lane ends with every
+  // total); lane j keeps pairs j and j + 64 and writes them, so the rows are
+  // written by the wavefront in parallel.
+  constexpr int NSLOT = (NBM + 63) / 64;
+  float myr[NSLOT], myi[NSLOT];
+#pragma unroll
+  for (int u = 0; u < NSLOT; ++u) myr[u] = myi[u] = 0.f;
 #pragma unroll
   for (int q = 0; q < NBM; ++q) {
     float vr = ar[q], vi = ai[q];
@@ -252,36 +265,39 @@ __global__ __launch_bounds__(256) void k_forward_snap(const PixPack* __restrict_
       vr += __shfl_xor(vr, o, 64);
       vi += __shfl_xor(vi, o, 64);
     }
-    ar[q] = vr;
-    ai[q] = vi;
+    if (lane == (q & 63)) {
+      myr[q >> 6] = vr;
+      myi[q >> 6] = vi;
+    }
   }
-  if (lane == 0) {
-    const int NB = N * (N - 1) / 2;
-    const int64_t K = S * NB;
-    int pm = 0;
+  const int NB = N * (N - 1) / 2;
+  const int64_t K = S * NB;
 #pragma unroll
-    for (int a = 0; a < NMAX - 1; ++a) {
-#pragma unroll
-      for (int b = a + 1; b < NMAX; ++b, ++pm) {
-        if (b < N) {
-          const int64_t k = s * NB + (a * N - a * (a + 1) / 2 + (b - a - 1));
-          double vr = ar[pm], vi = ai[pm];
-          if (track) {  // V e^{+2 pi i w}, w = (z_a - z_b) / lambda
-            const double w = (R[a].z - R[b].z) * inv_lambda;
-            double sn, cs;
-            sincospi(2.0 * (w - rint(w)), &sn, &cs);
-            const double nr = vr * cs - vi * sn, ni = vr * sn + vi * cs;
-            vr = nr;
-            vi = ni;
-          }
-          unsafeAtomicAdd(&V[2 * k], vr);
-          unsafeAtomicAdd(&V[2 * k + 1], vi);
-          if (herm) {
-            unsafeAtomicAdd(&V[2 * (k + K)], vr);
-            unsafeAtomicAdd(&V[2 * (k + K) + 1], -vi);
-          }
-        }
-      }
+  for (int u = 0; u < NSLOT; ++u) {
+    int q = u * 64 + lane;  // padded pair index: pairs (a, b) of NMAX satellites, a < b, row-major
+    if (q >= NBM) break;
+    int a = 0;
+    while (q >= NMAX - 1 - a) {
+      q -= NMAX - 1 - a;
+      ++a;
+    }
+    const int b = a + 1 + q;
+    if (b >= N) continue;  // a padding satellite
+    const int64_t k = s * NB + (a * N - a * (a + 1) / 2 + (b - a - 1));
+    double vr = myr[u], vi = myi[u];
+    if (track) {  // V e^{+2 pi i w}, w = (z_a - z_b) / lambda
+      const double w = (R[a].z - R[b].z) * inv_lambda;
+      double sn, cs;
+      sincospi(2.0 * (w - rint(w)), &sn, &cs);
+      const double nr = vr * cs - vi * sn, ni = vr * sn + vi * cs;
+      vr = nr;
+      vi = ni;
+    }
+    unsafeAtomicAdd(&V[2 * k], vr);
+    unsafeAtomicAdd(&V[2 * k + 1], vi);
+    if (herm) {
+      unsafeAtomicAdd(&V[2 * (k + K)], vr);
+      unsafeAtomicAdd(&V[2 * (k + K) + 1], -vi);
     }
   }
 }
@@ -326,8 +342,8 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
   SatCls* cls = (SatCls*)(base + b_rec);
   float2* yw = (float2*)(base + ((b_rec + b_cls + 255) / 256) * 256);
   const double inv_lambda = 1.0 / wavelength;
-  k_snap_sats<<<grid_rows(SN), 256, 0, s>>>(sat_pos, SN, mask ? geom->R_moon : 0.0, mask ? geom->orbit_radius : 0.0,
-                                            mask ? 1 : 0, rec, cls);
+  k_snap_sats<<<grid_rows(SN), 256, 0, s>>>(sat_pos, S, N, N, mask ? geom->R_moon : 0.0,
+                                            mask ? geom->orbit_radius : 0.0, mask ? 1 : 0, rec, cls);
   VISS_LAUNCHED(k_snap_sats);
   k_snap_rows<<<grid_rows(K), 256, 0, s>>>(sat_pos, y, wt, S, N, inv_lambda, (flags & VISS_PHASE_TRACK) ? 1 : 0, yw);
   VISS_LAUNCHED(k_snap_rows);
@@ -372,15 +388,16 @@ extern "C" int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const 
   hipStream_t s = as_stream(stream);
   rc = sky_set_brightness(sky, I, false, s);
   if (rc != VISS_OK) return rc;
-  const int64_t SN = S * N;
-  const size_t b_rec = sizeof(SatRec) * SN, b_cls = sizeof(SatCls) * SN;
+  const int NP = N <= 4 ? 4 : N <= 8 ? 8 : N <= 10 ? 10 : N <= 12 ? 12 : 16;  // the kernel's NMAX
+  const int64_t SNP = S * NP;
+  const size_t b_rec = sizeof(SatRec) * SNP, b_cls = sizeof(SatCls) * SNP;
   rc = sky->ws.reserve(b_rec + b_cls + 256, s);
   if (rc != VISS_OK) return rc;
   SatRec* rec = (SatRec*)sky->ws.p;
   SatCls* cls = (SatCls*)((char*)sky->ws.p + b_rec);
   const double inv_lambda = 1.0 / wavelength;
-  k_snap_sats<<<grid_rows(SN), 256, 0, s>>>(sat_pos, SN, mask ? geom->R_moon : 0.0, mask ? geom->orbit_radius : 0.0,
-                                            mask ? 1 : 0, rec, cls);
+  k_snap_sats<<<grid_rows(SNP), 256, 0, s>>>(sat_pos, S, N, NP, mask ? geom->R_moon : 0.0,
+                                             mask ? geom->orbit_radius : 0.0, mask ? 1 : 0, rec, cls);
   VISS_LAUNCHED(k_snap_sats);
   const int64_t bx = cdiv_s(S, 4);
   // 256 tiles (16,384 pixels, 512 KB of pixel data) per range, fewer when the
@@ -394,13 +411,13 @@ extern "C" int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const 
 #define VISS_FS(M, NM)                                                                                             \
   k_forward_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->cone, sky->ntile, tpr, rec, cls, S, N, inv_lambda, track, \
                                              herm, V)
-  if (N <= 4) {
+  if (NP == 4) {
     if (mask) VISS_FS(true, 4); else VISS_FS(false, 4);
-  } else if (N <= 8) {
+  } else if (NP == 8) {
     if (mask) VISS_FS(true, 8); else VISS_FS(false, 8);
-  } else if (N <= 10) {
+  } else if (NP == 10) {
     if (mask) VISS_FS(true, 10); else VISS_FS(false, 10);
-  } else if (N <= 12) {
+  } else if (NP == 12) {
     if (mask) VISS_FS(true, 12); else VISS_FS(false, 12);
   } else {
     if (mask) VISS_FS(true, 16); else VISS_FS(false, 16);
