@@ -63,22 +63,38 @@ __global__ void k_snap_sats(const double* __restrict__ pos, int64_t S, int N, in
   }
 }
 
-// Per row: y' = wt * y [* exp(-2 pi i w)], w = (z_a - z_b) / lambda, as FP32 complex.
+// Padded pair slot q in [0, NP (NP - 1) / 2) -> satellites (a, b), a < b < NP,
+// in row-major order (the order of a snapshot's rows when NP = N).
+__device__ __forceinline__ void pair_of(int q, int NP, int& a, int& b) {
+  a = 0;
+  while (q >= NP - 1 - a) {
+    q -= NP - 1 - a;
+    ++a;
+  }
+  b = a + 1 + q;
+}
+
+// Row of pair (a, b) within a snapshot of N satellites.
+__device__ __forceinline__ int row_in_snapshot(int a, int b, int N) { return a * N - a * (a + 1) / 2 + (b - a - 1); }
+
+// Per padded pair slot: y' = wt * y [* exp(-2 pi i w)], w = (z_a - z_b) / lambda,
+// as FP32 complex; 0 for the slots of a padding satellite.
 __global__ void k_snap_rows(const double* __restrict__ pos, const double* __restrict__ y,
-                            const double* __restrict__ wt, int64_t S, int N, double inv_lambda, int track,
+                            const double* __restrict__ wt, int64_t S, int N, int NP, double inv_lambda, int track,
                             float2* __restrict__ yw) {
-  const int NB = N * (N - 1) / 2;
-  const int64_t K = S * NB;
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
+  const int NB = N * (N - 1) / 2, NBP = NP * (NP - 1) / 2;
+  const int64_t J = S * NBP;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < J; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = j / NBP;
+    int a, b;
+    pair_of((int)(j - s * NBP), NP, a, b);
+    if (b >= N) {
+      yw[j] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const int64_t k = s * NB + row_in_snapshot(a, b, N);
     double yr = y[2 * k], yi = y[2 * k + 1];
     if (track) {
-      const int64_t s = k / NB;
-      int q = (int)(k - s * NB), a = 0;
-      while (q >= N - 1 - a) {
-        q -= N - 1 - a;
-        ++a;
-      }
-      const int b = a + 1 + q;
       const double w = (pos[3 * (s * N + a) + 2] - pos[3 * (s * N + b) + 2]) * inv_lambda;
       double sn, cs;
       sincospi(-2.0 * (w - rint(w)), &sn, &cs);
@@ -90,12 +106,88 @@ __global__ void k_snap_rows(const double* __restrict__ pos, const double* __rest
       yr *= wt[k];
       yi *= wt[k];
     }
-    yw[k] = make_float2((float)yr, (float)yi);
+    yw[j] = make_float2((float)yr, (float)yi);
   }
 }
 
+// Conservative cone test of the N real satellites of one snapshot against a
+// tile: any_v = some satellite may see some pixel, all_v = every satellite
+// sees every pixel.
+template <int NMAX>
+__device__ __forceinline__ void classify_snapshot(const float4 cn, const SatCls* __restrict__ c, int N, bool& any_v,
+                                                  bool& all_v) {
+  any_v = false;
+  all_v = true;
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    if (i < N) {
+      const SatCls q = c[i];
+      const float d = fmaf(cn.x, q.ax, fmaf(cn.y, q.ay, cn.z * q.az));
+      any_v |= d + cn.w >= q.c - kClsEps;
+      all_v &= d - cn.w >= q.c + kClsEps;
+    }
+  }
+}
+
+// Satellite phasors of PPL pixels: G_i = [s.r_i >= t_i] exp(+2 pi i r_i.s / lambda).
+// VIS = false: every satellite sees every pixel (no test).
+template <bool VIS, int NMAX, int PPL>
+__device__ __forceinline__ void snapshot_phasors(const SatRec* __restrict__ R, const double (&l)[PPL],
+                                                 const double (&m)[PPL], const double (&n)[PPL], double inv_lambda,
+                                                 float (&gr)[NMAX][PPL], float (&gi)[NMAX][PPL]) {
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    const SatRec q = R[i];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) {
+      const double D = fma(q.x, l[p], fma(q.y, m[p], q.z * n[p]));
+      float sn, cs;
+      sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
+      const bool vis = !VIS || (D >= q.t);
+      gr[i][p] = vis ? cs : 0.f;
+      gi[i][p] = vis ? sn : 0.f;
+    }
+  }
+}
+
+// One snapshot's contribution to PPL pixels: Re sum_a G_a H_a, H_a = sum_{b>a} y_ab conj(G_b).
+template <bool VIS, int NMAX, int PPL>
+__device__ __forceinline__ void bp_snapshot(const SatRec* __restrict__ R, const float2* __restrict__ Y,
+                                            const double (&l)[PPL], const double (&m)[PPL], const double (&n)[PPL],
+                                            double inv_lambda, double (&acc)[PPL]) {
+  float gr[NMAX][PPL], gi[NMAX][PPL];
+  snapshot_phasors<VIS, NMAX, PPL>(R, l, m, n, inv_lambda, gr, gi);
+  float part[PPL];
+#pragma unroll
+  for (int p = 0; p < PPL; ++p) part[p] = 0.f;
+  int pr = 0;
+#pragma unroll
+  for (int a = 0; a < NMAX - 1; ++a) {
+    float hr[PPL], hi[PPL];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) hr[p] = hi[p] = 0.f;
+#pragma unroll
+    for (int b = a + 1; b < NMAX; ++b) {
+      const float2 yv = Y[pr++];
+#pragma unroll
+      for (int p = 0; p < PPL; ++p) {  // H += y conj(G_b)
+        hr[p] = fmaf(yv.x, gr[b][p], fmaf(yv.y, gi[b][p], hr[p]));
+        hi[p] = fmaf(yv.y, gr[b][p], fmaf(-yv.x, gi[b][p], hi[p]));
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) part[p] = fmaf(gr[a][p], hr[p], fmaf(-gi[a][p], hi[p], part[p]));
+  }
+#pragma unroll
+  for (int p = 0; p < PPL; ++p) acc[p] += (double)part[p];
+}
+
 // Grid: x = ceil(nbtile / 4) blocks of 4 wavefronts (one kBpTile-pixel tile per
-// wavefront), y = snapshot chunks.  NMAX >= N bounds the register arrays.
+// wavefront), y = snapshot chunks.  Records and rows are padded to NMAX
+// satellites per snapshot (padding rows carry y = 0), so the pair loops have
+// no per-satellite branches.  Snapshots are classified against the tile 64
+// at a time, one per lane; those every satellite sees whole skip the
+// per-pixel occultation test, those no satellite sees are skipped.
 template <bool MASK, int NMAX>
 __global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restrict__ pix,
                                                           const float4* __restrict__ bcone, int64_t nbtile,
@@ -105,6 +197,7 @@ __global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restr
                                                           double inv_lambda, int64_t s_per_chunk, double herm,
                                                           double* __restrict__ x) {
   constexpr int PPL = kBpPixPerLane;
+  constexpr int NBP = NMAX * (NMAX - 1) / 2;
   const int lane = threadIdx.x & 63;
   const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
   if (tile >= nbtile) return;  // wave-uniform
@@ -118,91 +211,81 @@ __global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restr
     n[i] = q.n;
     idx[i] = q.idx;
   }
-  const float4 cn = bcone[tile];
-  const int NB = N * (N - 1) / 2;
+  float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
+  if (MASK) cn = bcone[tile];
   const int64_t s0 = (int64_t)blockIdx.y * s_per_chunk;
   const int64_t s1 = min(S, s0 + s_per_chunk);
   double acc[PPL];
 #pragma unroll
   for (int i = 0; i < PPL; ++i) acc[i] = 0.0;
-  for (int64_t s = s0; s < s1; ++s) {
-    const SatRec* __restrict__ R = rec + s * N;
+  for (int64_t sb = s0; sb < s1; sb += 64) {
+    const int ns = (int)min((int64_t)64, s1 - sb);
+    uint64_t anym, fullm;
     if (MASK) {
-      // skip the snapshot when no satellite can see any pixel of this tile
-      bool any = false;
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i) {
-        if (i < N) {
-          const SatCls c = cls[s * N + i];
-          const float d = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
-          any |= d + cn.w >= c.c - kClsEps;
-        }
-      }
-      if (__ballot(any) == 0) continue;  // every lane agrees: tile and snapshot are wave-uniform
+      bool any_v = false, all_v = false;
+      if (lane < ns) classify_snapshot<NMAX>(cn, cls + (sb + lane) * NMAX, N, any_v, all_v);
+      anym = __ballot(any_v);
+      fullm = __ballot(all_v);
+    } else {
+      anym = ns == 64 ? ~0ull : ((1ull << ns) - 1ull);
+      fullm = anym;
     }
-    float gr[NMAX][PPL], gi[NMAX][PPL];
-#pragma unroll
-    for (int i = 0; i < NMAX; ++i) {
-      if (i < N) {
-        const SatRec q = R[i];
-#pragma unroll
-        for (int p = 0; p < PPL; ++p) {
-          const double D = fma(q.x, l[p], fma(q.y, m[p], q.z * n[p]));
-          float sn, cs;
-          sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
-          const bool vis = !MASK || (D >= q.t);
-          gr[i][p] = vis ? cs : 0.f;
-          gi[i][p] = vis ? sn : 0.f;
-        }
-      }
+    // fully visible snapshots, then partly visible ones: two loops rather
+    // than a branch per snapshot, so the accumulators stay in place
+    uint64_t fm = anym & fullm;
+    while (fm) {
+      const int64_t s = sb + __builtin_ctzll(fm);
+      fm &= fm - 1ull;
+      bp_snapshot<false, NMAX, PPL>(rec + s * NMAX, yw + s * NBP, l, m, n, inv_lambda, acc);
     }
-    const float2* __restrict__ Y = yw + s * NB;
-    float part[PPL];
-#pragma unroll
-    for (int p = 0; p < PPL; ++p) part[p] = 0.f;
-    int pr = 0;
-#pragma unroll
-    for (int a = 0; a < NMAX - 1; ++a) {
-      if (a < N - 1) {
-        float hr[PPL], hi[PPL];
-#pragma unroll
-        for (int p = 0; p < PPL; ++p) hr[p] = hi[p] = 0.f;
-#pragma unroll
-        for (int b = a + 1; b < NMAX; ++b) {
-          if (b < N) {
-            const float2 yv = Y[pr++];
-#pragma unroll
-            for (int p = 0; p < PPL; ++p) {  // H += y conj(G_b)
-              hr[p] = fmaf(yv.x, gr[b][p], fmaf(yv.y, gi[b][p], hr[p]));
-              hi[p] = fmaf(yv.y, gr[b][p], fmaf(-yv.x, gi[b][p], hi[p]));
-            }
-          }
-        }
-#pragma unroll
-        for (int p = 0; p < PPL; ++p) part[p] = fmaf(gr[a][p], hr[p], fmaf(-gi[a][p], hi[p], part[p]));
-      }
+    uint64_t pm = MASK ? (anym & ~fullm) : 0ull;
+    while (pm) {
+      const int64_t s = sb + __builtin_ctzll(pm);
+      pm &= pm - 1ull;
+      bp_snapshot<true, NMAX, PPL>(rec + s * NMAX, yw + s * NBP, l, m, n, inv_lambda, acc);
     }
-#pragma unroll
-    for (int p = 0; p < PPL; ++p) acc[p] += (double)part[p];
   }
 #pragma unroll
   for (int p = 0; p < PPL; ++p)
     if (idx[p] >= 0) unsafeAtomicAdd(&x[idx[p]], herm * acc[p]);
 }
 
+// One 64-pixel tile's contribution to a snapshot's pair sums (one pixel per
+// lane): (ar, ai)[pair a<b] += I conj(G_a) G_b.
+template <bool VIS, int NMAX>
+__device__ __forceinline__ void fwd_snapshot_tile(const PixPack& q, const SatRec* __restrict__ R, double inv_lambda,
+                                                  float (&ar)[NMAX * (NMAX - 1) / 2],
+                                                  float (&ai)[NMAX * (NMAX - 1) / 2]) {
+  const double l[1] = {q.l}, m[1] = {q.m}, n[1] = {q.n};
+  float gr[NMAX][1], gi[NMAX][1];
+  snapshot_phasors<VIS, NMAX, 1>(R, l, m, n, inv_lambda, gr, gi);
+  int pr = 0;
+#pragma unroll
+  for (int a = 0; a < NMAX - 1; ++a) {
+    const float fr = q.I * gr[a][0], fi = -q.I * gi[a][0];  // I conj(G_a)
+#pragma unroll
+    for (int b = a + 1; b < NMAX; ++b, ++pr) {  // F_a G_b
+      ar[pr] = fmaf(fr, gr[b][0], fmaf(-fi, gi[b][0], ar[pr]));
+      ai[pr] = fmaf(fr, gi[b][0], fmaf(fi, gr[b][0], ai[pr]));
+    }
+  }
+}
 
 // Forward over snapshot-structured rows: V_ab(s) = sum_p I_p conj(G_a(p)) G_b(p)
 // (= sum_p I_p m_kp exp(-2 pi i b . s_p)).  One wavefront per (snapshot,
 // pixel range); each lane owns one pixel of every 64-pixel tile of the range
-// and keeps FP32 partial sums for every pair of the snapshot in registers;
-// at the end of the range the partials are summed across the wavefront
-// (butterfly) and added into V with FP64 atomics.
+// and keeps FP32 partial sums for every pair of the snapshot in registers.
+// Tiles are classified against the snapshot 64 at a time, one per lane.  At
+// the end of the range the partials are summed across the wavefront and added
+// into V with FP64 atomics.  Records are padded to NMAX satellites per
+// snapshot: pairs with a padding satellite are accumulated and dropped, which
+// keeps per-satellite branches out of the pair loop.  NMAX <= 10 holds the
+// 90 FP32 partial sums in at most 168 VGPRs, 3 wavefronts per SIMD.
 template <bool MASK, int NMAX>
-__global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(const PixPack* __restrict__ pix, const float4* __restrict__ cone,
-                                                      int64_t ntile, int64_t tiles_per_range,
-                                                      const SatRec* __restrict__ rec, const SatCls* __restrict__ cls,
-                                                      int64_t S, int N, double inv_lambda, int track, int herm,
-                                                      double* __restrict__ V) {
+__global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(
+    const PixPack* __restrict__ pix, const float4* __restrict__ cone, int64_t ntile, int64_t tiles_per_range,
+    const SatRec* __restrict__ rec, const SatCls* __restrict__ cls, int64_t S, int N, double inv_lambda, int track,
+    int herm, double* __restrict__ V) {
   constexpr int NBM = NMAX * (NMAX - 1) / 2;
   const int lane = threadIdx.x & 63;
   const int64_t s = (int64_t)blockIdx.x * 4 + wave_in_block();
@@ -213,41 +296,29 @@ __global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(const 
   float ar[NBM], ai[NBM];
 #pragma unroll
   for (int q = 0; q < NBM; ++q) ar[q] = ai[q] = 0.f;
-  for (int64_t t = t0; t < t1; ++t) {
+  for (int64_t tb = t0; tb < t1; tb += 64) {
+    const int nt = (int)min((int64_t)64, t1 - tb);
+    uint64_t anym, fullm;
     if (MASK) {
-      const float4 cn = cone[t];
-      bool any = false;
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i) {  // padding records never see a tile
-        const SatCls c = cls[s * NMAX + i];
-        const float d = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
-        any |= d + cn.w >= c.c - kClsEps;
-      }
-      if (__ballot(any) == 0) continue;  // no satellite sees this tile
+      bool any_v = false, all_v = false;
+      if (lane < nt) classify_snapshot<NMAX>(cone[tb + lane], cls + s * NMAX, N, any_v, all_v);
+      anym = __ballot(any_v);
+      fullm = __ballot(all_v);
+    } else {
+      anym = nt == 64 ? ~0ull : ((1ull << nt) - 1ull);
+      fullm = anym;
     }
-    const PixPack q = pix[t * kTile + lane];
-    // Every record up to NMAX is evaluated (no per-satellite branches in the
-    // loop); pairs with a padding satellite are accumulated and dropped.
-    float gr[NMAX], gi[NMAX];
-#pragma unroll
-    for (int i = 0; i < NMAX; ++i) {
-      const SatRec r = R[i];
-      const double D = fma(r.x, q.l, fma(r.y, q.m, r.z * q.n));
-      float sn, cs;
-      sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
-      const bool vis = !MASK || (D >= r.t);
-      gr[i] = vis ? cs : 0.f;
-      gi[i] = vis ? sn : 0.f;
+    uint64_t fm = anym & fullm;
+    while (fm) {
+      const int64_t t = tb + __builtin_ctzll(fm);
+      fm &= fm - 1ull;
+      fwd_snapshot_tile<false, NMAX>(pix[t * kTile + lane], R, inv_lambda, ar, ai);
     }
-    int pr = 0;
-#pragma unroll
-    for (int a = 0; a < NMAX - 1; ++a) {
-      const float fr = q.I * gr[a], fi = -q.I * gi[a];  // I conj(G_a)
-#pragma unroll
-      for (int b = a + 1; b < NMAX; ++b, ++pr) {  // F_a G_b
-        ar[pr] = fmaf(fr, gr[b], fmaf(-fi, gi[b], ar[pr]));
-        ai[pr] = fmaf(fr, gi[b], fmaf(fi, gr[b], ai[pr]));
-      }
+    uint64_t pm = MASK ? (anym & ~fullm) : 0ull;
+    while (pm) {
+      const int64_t t = tb + __builtin_ctzll(pm);
+      pm &= pm - 1ull;
+      fwd_snapshot_tile<true, NMAX>(pix[t * kTile + lane], R, inv_lambda, ar, ai);
     }
   }
   // Sum over the wavefront's pixels (xor butterfly: every lane ends with every
@@ -274,16 +345,12 @@ __global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(const 
   const int64_t K = S * NB;
 #pragma unroll
   for (int u = 0; u < NSLOT; ++u) {
-    int q = u * 64 + lane;  // padded pair index: pairs (a, b) of NMAX satellites, a < b, row-major
+    const int q = u * 64 + lane;
     if (q >= NBM) break;
-    int a = 0;
-    while (q >= NMAX - 1 - a) {
-      q -= NMAX - 1 - a;
-      ++a;
-    }
-    const int b = a + 1 + q;
+    int a, b;
+    pair_of(q, NMAX, a, b);
     if (b >= N) continue;  // a padding satellite
-    const int64_t k = s * NB + (a * N - a * (a + 1) / 2 + (b - a - 1));
+    const int64_t k = s * NB + row_in_snapshot(a, b, N);
     double vr = myr[u], vi = myi[u];
     if (track) {  // V e^{+2 pi i w}, w = (z_a - z_b) / lambda
       const double w = (R[a].z - R[b].z) * inv_lambda;
@@ -303,6 +370,10 @@ __global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(const 
 }
 
 static int64_t cdiv_s(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Satellites per snapshot in the padded records: the kernels are instantiated
+// for these counts.
+static int padded_sats(int N) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 10 ? 10 : N <= 12 ? 12 : 16; }
 
 static unsigned grid_rows(int64_t n) {
   int64_t g = cdiv_s(n, 256);
@@ -333,8 +404,9 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
                  "viss_sky_backproject_snapshots: bad geometry");
   }
   hipStream_t s = as_stream(stream);
-  const int64_t NB = (int64_t)N * (N - 1) / 2, SN = S * N, K = S * NB;
-  const size_t b_rec = sizeof(SatRec) * SN, b_cls = sizeof(SatCls) * SN, b_y = sizeof(float2) * K;
+  const int NP = padded_sats(N);  // the kernel's NMAX
+  const int64_t NBP = (int64_t)NP * (NP - 1) / 2, SNP = S * NP, J = S * NBP;
+  const size_t b_rec = sizeof(SatRec) * SNP, b_cls = sizeof(SatCls) * SNP, b_y = sizeof(float2) * J;
   rc = sky->ws.reserve(b_rec + b_cls + b_y + 512, s);
   if (rc != VISS_OK) return rc;
   char* base = (char*)sky->ws.p;
@@ -342,14 +414,15 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
   SatCls* cls = (SatCls*)(base + b_rec);
   float2* yw = (float2*)(base + ((b_rec + b_cls + 255) / 256) * 256);
   const double inv_lambda = 1.0 / wavelength;
-  k_snap_sats<<<grid_rows(SN), 256, 0, s>>>(sat_pos, S, N, N, mask ? geom->R_moon : 0.0,
-                                            mask ? geom->orbit_radius : 0.0, mask ? 1 : 0, rec, cls);
+  k_snap_sats<<<grid_rows(SNP), 256, 0, s>>>(sat_pos, S, N, NP, mask ? geom->R_moon : 0.0,
+                                             mask ? geom->orbit_radius : 0.0, mask ? 1 : 0, rec, cls);
   VISS_LAUNCHED(k_snap_sats);
-  k_snap_rows<<<grid_rows(K), 256, 0, s>>>(sat_pos, y, wt, S, N, inv_lambda, (flags & VISS_PHASE_TRACK) ? 1 : 0, yw);
+  k_snap_rows<<<grid_rows(J), 256, 0, s>>>(sat_pos, y, wt, S, N, NP, inv_lambda, (flags & VISS_PHASE_TRACK) ? 1 : 0,
+                                           yw);
   VISS_LAUNCHED(k_snap_rows);
   const int64_t bx = cdiv_s(sky->nbtile, 4);
   // about 4096 rows per chunk, as the per-row kernel; more chunks for small launches
-  int64_t spc = std::max<int64_t>(1, 4096 / NB);
+  int64_t spc = std::max<int64_t>(1, 4096 / ((int64_t)N * (N - 1) / 2));
   while (spc > 1 && bx * cdiv_s(S, spc) < 4096) spc = (spc + 1) / 2;
   if (cdiv_s(S, spc) > 65535) spc = cdiv_s(S, 65535);
   dim3 grid((unsigned)bx, (unsigned)cdiv_s(S, spc));
@@ -357,8 +430,14 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
 #define VISS_BPS(M, NM)                                                                                             \
   k_backproject_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, rec, cls, yw, S, N, inv_lambda, \
                                                  spc, herm, x)
-  if (N <= 8) {
+  if (NP == 4) {
+    if (mask) VISS_BPS(true, 4); else VISS_BPS(false, 4);
+  } else if (NP == 8) {
     if (mask) VISS_BPS(true, 8); else VISS_BPS(false, 8);
+  } else if (NP == 10) {
+    if (mask) VISS_BPS(true, 10); else VISS_BPS(false, 10);
+  } else if (NP == 12) {
+    if (mask) VISS_BPS(true, 12); else VISS_BPS(false, 12);
   } else {
     if (mask) VISS_BPS(true, 16); else VISS_BPS(false, 16);
   }
@@ -388,7 +467,7 @@ extern "C" int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const 
   hipStream_t s = as_stream(stream);
   rc = sky_set_brightness(sky, I, false, s);
   if (rc != VISS_OK) return rc;
-  const int NP = N <= 4 ? 4 : N <= 8 ? 8 : N <= 10 ? 10 : N <= 12 ? 12 : 16;  // the kernel's NMAX
+  const int NP = padded_sats(N);  // the kernel's NMAX
   const int64_t SNP = S * NP;
   const size_t b_rec = sizeof(SatRec) * SNP, b_cls = sizeof(SatCls) * SNP;
   rc = sky->ws.reserve(b_rec + b_cls + 256, s);
