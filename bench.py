@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 GPU comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the pix2ang / weights HBM measurements")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched one-launch comparison")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched comparison (profiling runs of that workload)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the HBM-traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -87,13 +89,21 @@ def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
     ff = kib / max(copies_f)
     out = [f"calibration: {calib_mib} MiB copy reads FETCH_SIZE={max(copies_f):.1f} KiB, WRITE_SIZE={max(copies_w):.1f} KiB",
            f"scale factors: fetch x{ff:.4f}, write x{fw:.4f}"]
-    for key in ("k_forward<true, false>", "k_backproject<true, false>", "k_pix2ang", "k_shell_hist", "k_row_weights"):
-        f = [float(r["Counter_Value"]) for r in fr if key in r["Kernel_Name"]]
-        w = [float(r["Counter_Value"]) for r in wr if key in r["Kernel_Name"]]
-        if not f or not w:
-            continue
-        fb, wb = f[0] * 1024 * ff, w[0] * 1024 * fw
-        out.append(f"kernel={key.split('<')[0]} fetch_bytes={fb:.6e} write_bytes={wb:.6e} bytes_per_launch={fb + wb:.6e}")
+    keys = ("k_forward<true, false>", "k_backproject<true, false>", "k_forward_snap<true, 10>",
+            "k_backproject_snap<true, 10>", "k_forward_batch<true, false>", "k_pix2ang", "k_pixel_dirs",
+            "k_bp_prologue", "k_shell_hist", "k_row_weights")
+    for key in keys:
+        f = [r for r in fr if key in r["Kernel_Name"]]
+        w = [r for r in wr if key in r["Kernel_Name"]]
+        # one line per launch shape (same program order in both runs); the first is the workload's
+        seen = set()
+        for rf, rw in zip(f, w):
+            if rf["Grid_Size"] in seen:
+                continue
+            seen.add(rf["Grid_Size"])
+            fb, wb = float(rf["Counter_Value"]) * 1024 * ff, float(rw["Counter_Value"]) * 1024 * fw
+            out.append(f"kernel={key.split('<')[0]} fetch_bytes={fb:.6e} write_bytes={wb:.6e} "
+                       f"bytes_per_launch={fb + wb:.6e} grid={rf['Grid_Size']}")
     return "\n".join(out)
 
 
@@ -135,6 +145,9 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
         del src, dst
+    if args.batched_only:
+        print(json.dumps({"batched": batched(ops, geo, dev)}), flush=True)
+        return
     wl = geo.WORKLOADS[args.workload]
     log(f"[bench] generating {wl.name}: nside={wl.nside} K={wl.n_vis}")
     inp = geo.generate(wl)
@@ -237,6 +250,8 @@ def main():
                            "speedup_mixed_over_fp64": value / v64, "rel_l2_forward": rel_v,
                            "rel_l2_adjoint": rel_x, "max_err_forward_rel_to_max": max_v}
         log(f"[bench] fp64 GPU: {w64 * 1e3:.1f} ms/step; mixed/fp64 {value / v64:.2f}x; rel L2 fwd {rel_v:.2e} adj {rel_x:.2e}")
+    V64 = V.clone() if not args.no_fp64 else None  # the all-FP64 results, for the snapshot paths' errors
+    x64 = x.clone() if not args.no_fp64 else None
 
     # snapshot-structured adjoint on the same rows (one phasor per satellite and pixel)
     pos = torch.from_numpy(inp.positions).to(dev)
@@ -255,6 +270,9 @@ def main():
         "ms": snap_ms, "pairs_per_s": visible / (snap_ms * 1e-3), "speedup_over_per_row_adjoint": adj_avg / snap_ms,
         "rel_l2_vs_per_row_adjoint": float(torch.linalg.vector_norm(xs - x_mixed) / torch.linalg.vector_norm(x_mixed)),
         "fwd_plus_snapshot_adjoint_pairs_per_s": 2 * visible / ((fwd_avg + snap_ms) * 1e-3)}
+    if x64 is not None:
+        out["snapshot_adjoint"]["rel_l2_vs_fp64_gpu"] = float(torch.linalg.vector_norm(xs - x64) /
+                                                             torch.linalg.vector_norm(x64))
     log(f"[bench] snapshot adjoint: {snap_ms:.1f} ms ({adj_avg / snap_ms:.2f}x the per-row adjoint)")
     Vs = torch.zeros_like(V_mixed)
     sky.forward_snapshots(I, pos, wl.wavelength, mask=True, R_moon=R, out=Vs)
@@ -270,6 +288,9 @@ def main():
         "ms": sf_ms, "pairs_per_s": visible / (sf_ms * 1e-3), "speedup_over_per_row_forward": fwd_avg / sf_ms,
         "rel_l2_vs_per_row_forward": float(torch.linalg.vector_norm(Vs - V_mixed) / torch.linalg.vector_norm(V_mixed)),
         "fwd_plus_adjoint_pairs_per_s": 2 * visible / ((sf_ms + snap_ms) * 1e-3)}
+    if V64 is not None:
+        out["snapshot_forward"]["rel_l2_vs_fp64_gpu"] = float(torch.linalg.vector_norm(Vs - V64) /
+                                                             torch.linalg.vector_norm(V64))
     log(f"[bench] snapshot forward: {sf_ms:.1f} ms ({fwd_avg / sf_ms:.2f}x the per-row forward)")
 
     if not args.no_cpu:

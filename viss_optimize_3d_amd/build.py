@@ -19,10 +19,13 @@ BIN = os.path.join(PKG, "bin")
 
 ARCH = os.environ.get("VISS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-# Packed FP32 instructions are disabled for all device code ("build flags").
+# Extra device flags (e.g. -DVISS_BP_PIX_PER_LANE=4) for launch-shape experiments.
+EXTRA = os.environ.get("VISS_EXTRA_FLAGS", "").split()
+# Packed FP32 instructions (v_pk_fma_f32 and the like) are disabled for all
+# device code; hipcc's host pass warns that it does not know the feature.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE, "-I", CSRC,
-          "-Wall", "-Wno-unused-function"] + DEVICE_FLAGS
+          "-Wall", "-Wno-unused-function"] + DEVICE_FLAGS + EXTRA
 
 SOURCES = ["viss_sky.hip", "viss_pix2ang.hip", "viss_nudft.hip", "viss_weights.hip", "viss_batch.hip", "viss_image.hip", "viss_snapshots.hip"]
 HOST_PROGRAMS = {"pix2ang": "host/pix2ang_main.cpp", "vissgen3d": "host/vissgen3d_main.cpp",

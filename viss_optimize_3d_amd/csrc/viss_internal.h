@@ -71,7 +71,10 @@ static_assert(sizeof(VisCls) == 32, "VisCls is 32 B");
 constexpr float kClsEps = 1e-5f;
 
 constexpr int kTile = 64;       // forward tile (pixels)
-constexpr int kBpPixPerLane = 2; // back-projection pixels per lane (measured best of 1, 2, 4, 8)
+#ifndef VISS_BP_PIX_PER_LANE
+#define VISS_BP_PIX_PER_LANE 2
+#endif
+constexpr int kBpPixPerLane = VISS_BP_PIX_PER_LANE;  // back-projection pixels per lane (measured best of 1, 2, 4, 8)
 constexpr int kBpTile = 64 * kBpPixPerLane;
 constexpr int kPadTo = kBpTile;  // sky padding granularity
 

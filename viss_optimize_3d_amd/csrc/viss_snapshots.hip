@@ -15,6 +15,7 @@
 // b . s up to FP64 rounding, and the reductions are the same blocked FP32 ->
 // FP64 sums.
 #include <algorithm>
+#include <cstdlib>
 
 #include "viss_kernels.h"
 
@@ -421,8 +422,14 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
                                            yw);
   VISS_LAUNCHED(k_snap_rows);
   const int64_t bx = cdiv_s(sky->nbtile, 4);
-  // about 4096 rows per chunk, as the per-row kernel; more chunks for small launches
-  int64_t spc = std::max<int64_t>(1, 4096 / ((int64_t)N * (N - 1) / 2));
+  // about 4096 rows per chunk, as the per-row kernel (VISS_BP_ROWS overrides it
+  // for launch-shape sweeps); more chunks for small launches
+  int64_t rows_per_chunk = 4096;
+  if (const char* e = getenv("VISS_BP_ROWS")) {
+    const long long v = atoll(e);
+    if (v > 0) rows_per_chunk = v;
+  }
+  int64_t spc = std::max<int64_t>(1, rows_per_chunk / ((int64_t)N * (N - 1) / 2));
   while (spc > 1 && bx * cdiv_s(S, spc) < 4096) spc = (spc + 1) / 2;
   if (cdiv_s(S, spc) > 65535) spc = cdiv_s(S, 65535);
   dim3 grid((unsigned)bx, (unsigned)cdiv_s(S, spc));
@@ -480,8 +487,13 @@ extern "C" int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const 
   VISS_LAUNCHED(k_snap_sats);
   const int64_t bx = cdiv_s(S, 4);
   // 256 tiles (16,384 pixels, 512 KB of pixel data) per range, fewer when the
-  // snapshots alone would not fill the device
+  // snapshots alone would not fill the device (VISS_SNAP_TILES overrides the
+  // starting length for launch-shape sweeps)
   int64_t tpr = 256;
+  if (const char* e = getenv("VISS_SNAP_TILES")) {
+    const long long v = atoll(e);
+    if (v > 0) tpr = v;
+  }
   while (tpr > 16 && bx * cdiv_s(sky->ntile, tpr) < 4096) tpr /= 2;
   tpr = std::max(tpr, cdiv_s(sky->ntile, 65535));
   VISS_REQUIRE(bx <= INT32_MAX, "viss_sky_forward_snapshots: too many snapshots");
