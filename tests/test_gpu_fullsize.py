@@ -1,0 +1,106 @@
+"""The workloads at their full sizes, checked through properties that do not
+need the oracle to finish the whole problem:
+
+  * blockage, nside 64, 280,000 visibilities (8 satellites x 10,000 rotating
+    snapshots, 1 MHz): mixed and all-FP64 forward / back-projection against
+    the FP64 oracle on a 5% sample of the rows (14,000 rows x 49,152 pixels);
+  * precision, nside 256, 900,000 visibilities (10 satellites x 20,000
+    snapshots, 10 MHz, sigma 50 km): the mixed kernels against the all-FP64
+    kernels over every row and pixel (7.1e11 pair evaluations each way), the
+    FP64 forward against the oracle on 256 rows, and the adjoint identity
+    <A x, y> = <x, A^H y> of the mixed pair at full size.
+
+Tolerances: mixed vs FP64 (GPU or oracle) relative L2 <= 1e-5 (the precision
+target; measured values near 3e-7 are printed); all-FP64 GPU vs the oracle
+<= 1e-11; the adjoint identity to 1e-6 relative (both sides are mixed-precision
+sums of 5.3e11 terms).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.cases import rel_l2
+from viss_optimize_3d_amd import geometry as geo
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+R = geo.R_MOON
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from viss_optimize_3d_amd import ops as _ops
+
+    return _ops
+
+
+def _setup(ops, name):
+    wl = geo.WORKLOADS[name]
+    inp = geo.generate(wl)
+    dev = torch.device("cuda:0")
+    th, ph = ops.pix2ang(wl.nside, nest=False, with_ring=False)
+    lmn, I = ops.pixel_dirs(th, ph, torch.from_numpy(inp.B).to(dev))
+    sky = ops.Sky(lmn)
+    rows = tuple(torch.from_numpy(a).to(dev) for a in (inp.uvw, inp.xyz_a, inp.xyz_b))
+    return wl, inp, lmn, I, sky, rows
+
+
+def test_blockage_workload_full_size(ops):
+    wl, inp, lmn, I, sky, (uvw, xa, xb) = _setup(ops, "blockage-nside64")
+    K = uvw.shape[1]
+    assert K == 280_000 and sky.P == 49_152
+    print(f"blockage-nside64: K={K} P={sky.P}", flush=True)
+    rng = np.random.default_rng(7)
+    sel = np.sort(rng.choice(K, K // 20, replace=False))
+    l, m, n = (lmn[i].cpu().numpy() for i in range(3))
+    Ih = I.cpu().numpy()
+    u, a, b = inp.uvw[:, sel], inp.xyz_a[:, sel], inp.xyz_b[:, sel]
+    Vo = oracle.forward(l, m, n, Ih, u, a, b, flags=oracle.MASK, R_moon=R)
+    y = (np.random.default_rng(8).standard_normal(sel.size) + 1j * np.random.default_rng(9).standard_normal(sel.size))
+    xo = oracle.backproject(l, m, n, u, y, None, a, b, flags=oracle.MASK, R_moon=R)
+    sel_t = torch.from_numpy(sel).cuda()
+    y_t = torch.from_numpy(y).cuda()
+    for fp64, tol in ((True, 1e-11), (False, 1e-5)):
+        V = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R, fp64=fp64)
+        rv = rel_l2(V[sel_t].cpu().numpy(), Vo)
+        x = sky.backproject(uvw[:, sel_t].contiguous(), y_t, None, xa[:, sel_t].contiguous(),
+                            xb[:, sel_t].contiguous(), mask=True, R_moon=R, fp64=fp64)
+        rx = rel_l2(x.cpu().numpy(), xo)
+        print(f"  fp64={fp64}: forward rel L2 {rv:.2e} on {sel.size} rows, back-projection {rx:.2e}", flush=True)
+        assert rv <= tol and rx <= tol
+
+
+def test_precision_workload_full_size(ops):
+    wl, inp, lmn, I, sky, (uvw, xa, xb) = _setup(ops, "precision-nside256")
+    K = uvw.shape[1]
+    assert K == 900_000 and sky.P == 786_432
+    print(f"precision-nside256: K={K} P={sky.P}", flush=True)
+    V64 = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R, fp64=True)
+    V32 = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R)
+    rv = float(torch.linalg.vector_norm(V32 - V64) / torch.linalg.vector_norm(V64))
+    # the FP64 kernel against the oracle on a row sample
+    sel = np.sort(np.random.default_rng(11).choice(K, 256, replace=False))
+    l, m, n = (lmn[i].cpu().numpy() for i in range(3))
+    Vo = oracle.forward(l, m, n, I.cpu().numpy(), inp.uvw[:, sel], inp.xyz_a[:, sel], inp.xyz_b[:, sel],
+                        flags=oracle.MASK, R_moon=R)
+    ro = rel_l2(V64[torch.from_numpy(sel).cuda()].cpu().numpy(), Vo)
+    # back-projection of the visibilities, mixed vs FP64
+    x64 = sky.backproject(uvw, V64, None, xa, xb, mask=True, R_moon=R, fp64=True)
+    x32 = sky.backproject(uvw, V64, None, xa, xb, mask=True, R_moon=R)
+    rx = float(torch.linalg.vector_norm(x32 - x64) / torch.linalg.vector_norm(x64))
+    # adjoint identity of the mixed pair: Re <A x, y> = <x, Re A^H y>
+    g = torch.Generator(device="cuda").manual_seed(5)
+    xr = torch.rand(sky.P, dtype=torch.float64, device="cuda", generator=g)
+    yr = torch.randn(K, dtype=torch.complex128, device="cuda", generator=g)
+    Ax = sky.forward(xr, uvw, xa, xb, mask=True, R_moon=R)
+    AHy = sky.backproject(uvw, yr, None, xa, xb, mask=True, R_moon=R)
+    lhs = float(torch.real(torch.sum(Ax * torch.conj(yr))))
+    rhs = float(torch.dot(xr, AHy))
+    rd = abs(lhs - rhs) / abs(rhs)
+    print(f"  forward mixed vs FP64 {rv:.2e}; FP64 vs oracle {ro:.2e} (256 rows); back-projection mixed vs FP64 "
+          f"{rx:.2e}; adjoint identity {rd:.2e}", flush=True)
+    assert rv <= 1e-5 and rx <= 1e-5
+    assert ro <= 1e-11
+    assert rd <= 1e-6

@@ -5,8 +5,10 @@
 // MATLAB tables with +1 (so every theta is NaN) and has no bounds guard.  Here
 // each thread holds its pixel's integers in registers and writes only the
 // outputs (16 B/pixel for theta, phi; +8 with ring/iphi): an HBM-store-bound
-// elementwise kernel.  The Morton de-interleave is done with bit masks, so
-// the 1024-entry pix2x/pix2y lookup table is not needed.
+// elementwise kernel.  theta is constant on a ring, so acos runs once per ring
+// (4 nside - 1 of them) and the other pixels copy it, L2-resident.  The Morton
+// de-interleave is done with bit masks, so the 1024-entry pix2x/pix2y lookup
+// table is not needed.
 //
 // Bit-exactness: z = (3n^2 - nr^2) / (3n^2) and phi = (pi/2) t / nr are single
 // correctly rounded FP64 operations on exact integers, identical to the host
@@ -91,27 +93,88 @@ __device__ inline RingPos ring_pos(int32_t p, int32_t nside) {
   return r;
 }
 
-#pragma clang fp contract(off)
-__device__ inline void ring_angles(const RingPos& r, int32_t nside, double& z, double& phi) {
-  const double three_ns2 = 3.0 * (double)(nside * nside);
-  if (r.jr < nside) {
-    z = (three_ns2 - (double)(r.nr * r.nr)) / three_ns2;
-  } else if (r.jr <= 3 * nside) {
-    z = (double)(2 * (2 * nside - r.jr)) / (double)(3 * nside);
+// Ring geometry of ring jr (1 .. 4 nside - 1): pixels in the ring and the
+// phase shift of its first pixel.
+__device__ inline void ring_shape(int32_t jr, int32_t nside, int32_t& nr, int32_t& kshift) {
+  if (jr < nside) {
+    nr = jr;
+    kshift = 0;
+  } else if (jr <= 3 * nside) {
+    nr = nside;
+    kshift = (jr - nside) & 1;
   } else {
-    z = -((three_ns2 - (double)(r.nr * r.nr)) / three_ns2);
+    nr = 4 * nside - jr;
+    kshift = 0;
   }
-  phi = (kPi / 2.0) * ((double)r.jp - 0.5 * (double)(r.kshift + 1)) / (double)r.nr;
 }
 
+// Spread the low 16 bits of v to the even bit positions (inverse of
+// compact_even_bits).
+__device__ inline uint32_t spread_even_bits(uint32_t v) {
+  v &= 0x0000ffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// The pixel of ring jr that holds the ring's theta: the first pixel of the
+// ring in RING order; in NEST order a pixel of face 0 (rings 1 .. 2n-1), 4
+// (ring 2n) or 8 (rings 2n+1 .. 4n-1) with ix + iy = jrll(face) n - jr - 1.
+__device__ inline int32_t ring_holder(int32_t jr, int32_t nside, int32_t order) {
+  if (order < 0) {
+    const int32_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    if (jr < nside) return 2 * jr * (jr - 1);
+    if (jr <= 3 * nside) return ncap + (jr - nside) * 4 * nside;
+    const int32_t i = 4 * nside - jr;
+    return npix - 2 * i * (i + 1);
+  }
+  const int32_t face = jr < 2 * nside ? 0 : (jr == 2 * nside ? 4 : 8);
+  const int32_t jrll = face == 0 ? 2 : (face == 4 ? 3 : 4);
+  const int32_t sum = jrll * nside - jr - 1;
+  const int32_t ix = sum < nside - 1 ? sum : nside - 1, iy = sum - ix;
+  return (face << (2 * order)) + (int32_t)(spread_even_bits((uint32_t)ix) | (spread_even_bits((uint32_t)iy) << 1));
+}
+
+#pragma clang fp contract(off)
+__device__ inline double ring_z(int32_t jr, int32_t nr, int32_t nside) {
+  const double three_ns2 = 3.0 * (double)(nside * nside);
+  if (jr < nside) return (three_ns2 - (double)(nr * nr)) / three_ns2;
+  if (jr <= 3 * nside) return (double)(2 * (2 * nside - jr)) / (double)(3 * nside);
+  return -((three_ns2 - (double)(nr * nr)) / three_ns2);
+}
+
+__device__ inline double ring_phi(const RingPos& r) {
+  return (kPi / 2.0) * ((double)r.jp - 0.5 * (double)(r.kshift + 1)) / (double)r.nr;
+}
+
+// theta is constant on a ring: acos(z) once per ring, written to the ring's
+// holder pixel (ring_holder); k_pix2ang copies it to the other pixels.
+__global__ void k_ring_theta(int32_t nside, int32_t order, double* __restrict__ theta) {
+  const int32_t nring = 4 * nside - 1;
+  for (int32_t jr = blockIdx.x * blockDim.x + threadIdx.x + 1; jr <= nring; jr += gridDim.x * blockDim.x) {
+    int32_t nr, kshift;
+    ring_shape(jr, nside, nr, kshift);
+    theta[ring_holder(jr, nside, order)] = acos(ring_z(jr, nr, nside));
+  }
+}
+
+// HOLDER: theta comes from the ring's holder pixel (k_ring_theta ran first);
+// otherwise every pixel computes acos itself (small maps, where the second
+// launch would cost more than the acos it saves).
+template <bool HOLDER>
 __global__ void k_pix2ang(int32_t nside, int32_t order, int32_t npix, double* __restrict__ theta,
                           double* __restrict__ phi, int32_t* __restrict__ ring, int32_t* __restrict__ iphi) {
   for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
     const RingPos r = order >= 0 ? nest_pos(p, nside, order) : ring_pos(p, nside);
-    double z, ph;
-    ring_angles(r, nside, z, ph);
-    theta[p] = acos(z);
-    phi[p] = ph;
+    if (HOLDER) {
+      const int32_t h = ring_holder(r.jr, nside, order);
+      if (p != h) theta[p] = theta[h];  // written by k_ring_theta, earlier on the stream
+    } else {
+      theta[p] = acos(ring_z(r.jr, r.nr, nside));
+    }
+    phi[p] = ring_phi(r);
     if (ring) ring[p] = r.jr;
     if (iphi) iphi[p] = r.jp;
   }
@@ -157,8 +220,14 @@ extern "C" int viss_pix2ang(int64_t nside, int scheme, double* theta, double* ph
     order = 0;
     while ((1ll << order) < nside) ++order;
   }
-  k_pix2ang<<<grid_1d(npix), 256, 0, as_stream(stream)>>>((int32_t)nside, order, (int32_t)npix, theta, phi, ring,
-                                                           iphi);
+  hipStream_t s = as_stream(stream);
+  if (npix >= (int64_t)1 << 22) {  // nside >= 592: one acos per ring
+    k_ring_theta<<<grid_1d(4 * nside - 1), 256, 0, s>>>((int32_t)nside, order, theta);
+    VISS_LAUNCHED(k_ring_theta);
+    k_pix2ang<true><<<grid_1d(npix), 256, 0, s>>>((int32_t)nside, order, (int32_t)npix, theta, phi, ring, iphi);
+  } else {
+    k_pix2ang<false><<<grid_1d(npix), 256, 0, s>>>((int32_t)nside, order, (int32_t)npix, theta, phi, ring, iphi);
+  }
   VISS_LAUNCHED(k_pix2ang);
   return VISS_OK;
 }

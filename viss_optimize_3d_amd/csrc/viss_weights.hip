@@ -10,6 +10,8 @@
 //   k_dcf        : dcf_j from the exact shell volume               [O(nr)]
 //   k_row_weights: dg_k, wt_k = fmin(dcf[gs_k] dg_k, 1/8)           [reads 16 B, writes 8-16 B per row]
 // All HBM-bound; the sort is dropped (its order is never used).
+#include <cstdint>
+
 #include "viss_internal.h"
 
 namespace viss {
@@ -23,6 +25,22 @@ __device__ inline int32_t shell_index(float bll) {
   return (int32_t)c;
 }
 
+__device__ inline void count_shell(int32_t g, int32_t nr, uint32_t* hist, uint32_t* mb, int use_lds,
+                                   unsigned long long& bad) {
+  if (g >= 1 && g <= nr) {
+    if (use_lds)
+      atomicAdd(&hist[g - 1], 1u);
+    else
+      atomicAdd(&mb[g - 1], 1u);
+  } else if (g != 0) {
+    ++bad;
+  }
+}
+
+// VEC: bll and gs are 16-B aligned and each thread takes 4 rows per step
+// (float4 in, int4 out): the LDS histogram allows only a few blocks per CU,
+// so the wider accesses are what keeps enough bytes in flight.
+template <bool VEC>
 __global__ void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t nr, int32_t* __restrict__ gs,
                              uint32_t* __restrict__ mb, unsigned long long* __restrict__ n_bad, int use_lds) {
   extern __shared__ uint32_t hist[];
@@ -31,17 +49,27 @@ __global__ void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t n
     __syncthreads();
   }
   unsigned long long bad = 0;
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t k_tail = 0;
+  if (VEC) {
+    const int64_t K4 = K / 4;
+    const float4* __restrict__ b4 = reinterpret_cast<const float4*>(bll);
+    int4* __restrict__ g4 = reinterpret_cast<int4*>(gs);
+    for (int64_t q = t0; q < K4; q += stride) {
+      const float4 b = b4[q];
+      const int4 g = make_int4(shell_index(b.x), shell_index(b.y), shell_index(b.z), shell_index(b.w));
+      g4[q] = g;
+      count_shell(g.x, nr, hist, mb, use_lds, bad);
+      count_shell(g.y, nr, hist, mb, use_lds, bad);
+      count_shell(g.z, nr, hist, mb, use_lds, bad);
+      count_shell(g.w, nr, hist, mb, use_lds, bad);
+    }
+    k_tail = K4 * 4;
+  }
+  for (int64_t k = k_tail + t0; k < K; k += stride) {
     const int32_t g = shell_index(bll[k]);
     gs[k] = g;
-    if (g >= 1 && g <= nr) {
-      if (use_lds)
-        atomicAdd(&hist[g - 1], 1u);
-      else
-        atomicAdd(&mb[g - 1], 1u);
-    } else if (g != 0) {
-      ++bad;
-    }
+    count_shell(g, nr, hist, mb, use_lds, bad);
   }
   if (n_bad) {
     for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
@@ -107,12 +135,23 @@ extern "C" int viss_weights(const float* bll, const double* w, int64_t K, int32_
   if (K > 0) {
     const size_t lds = (size_t)nr * sizeof(uint32_t);
     const int use_lds = lds <= (size_t)lds_max ? 1 : 0;
+    // as many resident blocks as the CUs' LDS holds (2..8 per CU): each block
+    // flushes its nr counters once, so no more than that
+    int lds_cu = 0;
+    VISS_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    int64_t per_cu = use_lds && lds > 0 ? (int64_t)lds_cu / (int64_t)lds : 8;
+    per_cu = per_cu < 2 ? 2 : (per_cu > 8 ? 8 : per_cu);
     int64_t blocks = (K + 256 * 16 - 1) / (256 * 16);
-    const int64_t cap = (int64_t)(ncu > 0 ? ncu : 256) * 2;
+    const int64_t cap = (int64_t)(ncu > 0 ? ncu : 256) * per_cu;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    k_shell_hist<<<(unsigned)blocks, 256, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
-                                                                   (unsigned long long*)n_out_of_range, use_lds);
+    const bool vec = ((uintptr_t)bll % 16 == 0) && ((uintptr_t)gs % 16 == 0);
+    if (vec)
+      k_shell_hist<true><<<(unsigned)blocks, 256, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
+                                                                         (unsigned long long*)n_out_of_range, use_lds);
+    else
+      k_shell_hist<false><<<(unsigned)blocks, 256, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
+                                                                          (unsigned long long*)n_out_of_range, use_lds);
     VISS_LAUNCHED(k_shell_hist);
   }
   k_dcf<<<(unsigned)((nr + 1 + 255) / 256 > 4096 ? 4096 : (nr + 1 + 255) / 256), 256, 0, s>>>(
