@@ -2,8 +2,10 @@
 need the oracle to finish the whole problem:
 
   * blockage, nside 64, 280,000 visibilities (8 satellites x 10,000 rotating
-    snapshots, 1 MHz): mixed and all-FP64 forward / back-projection against
-    the FP64 oracle on a 5% sample of the rows (14,000 rows x 49,152 pixels);
+    snapshots, 1 MHz): density-compensation weights over all rows against the
+    oracle (gs, mb, dcf bit-exact; wt 1e-14), then mixed and all-FP64 forward
+    and the density-compensated image against the FP64 oracle on a 5% sample
+    of the rows (14,000 rows x 49,152 pixels);
   * precision, nside 256, 900,000 visibilities (10 satellites x 20,000
     snapshots, 10 MHz, sigma 50 km): the mixed kernels against the all-FP64
     kernels over every row and pixel (7.1e11 pair evaluations each way), the
@@ -52,23 +54,34 @@ def test_blockage_workload_full_size(ops):
     K = uvw.shape[1]
     assert K == 280_000 and sky.P == 49_152
     print(f"blockage-nside64: K={K} P={sky.P}", flush=True)
+    # weights over every row (the histogram needs all of them)
+    bll = inp.bll.astype(np.float32)
+    nr = int(oracle.shell_index(bll).max())
+    out = ops.weights(torch.from_numpy(bll).cuda(), uvw[2].contiguous(), nr)
+    gs, mb, dcf, dg, wt, bad = oracle.weights(bll, inp.uvw[2], nr)
+    np.testing.assert_array_equal(out["gs"].cpu().numpy(), gs)
+    np.testing.assert_array_equal(out["mb"].cpu().numpy().astype(np.uint32), mb)
+    np.testing.assert_array_equal(out["dcf"].cpu().numpy(), dcf)
+    np.testing.assert_allclose(out["wt"].cpu().numpy(), wt, rtol=1e-14, atol=1e-13)
+    print(f"  weights: nr={nr}, gs / mb / dcf bit-exact over {K} rows", flush=True)
     rng = np.random.default_rng(7)
     sel = np.sort(rng.choice(K, K // 20, replace=False))
     l, m, n = (lmn[i].cpu().numpy() for i in range(3))
     Ih = I.cpu().numpy()
     u, a, b = inp.uvw[:, sel], inp.xyz_a[:, sel], inp.xyz_b[:, sel]
     Vo = oracle.forward(l, m, n, Ih, u, a, b, flags=oracle.MASK, R_moon=R)
-    y = (np.random.default_rng(8).standard_normal(sel.size) + 1j * np.random.default_rng(9).standard_normal(sel.size))
-    xo = oracle.backproject(l, m, n, u, y, None, a, b, flags=oracle.MASK, R_moon=R)
+    xo = oracle.backproject(l, m, n, u, Vo, wt[sel], a, b, flags=oracle.MASK, R_moon=R)
     sel_t = torch.from_numpy(sel).cuda()
-    y_t = torch.from_numpy(y).cuda()
+    Vo_t = torch.from_numpy(Vo).cuda()
+    wt_t = out["wt"][sel_t].contiguous()
     for fp64, tol in ((True, 1e-11), (False, 1e-5)):
         V = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R, fp64=fp64)
         rv = rel_l2(V[sel_t].cpu().numpy(), Vo)
-        x = sky.backproject(uvw[:, sel_t].contiguous(), y_t, None, xa[:, sel_t].contiguous(),
+        x = sky.backproject(uvw[:, sel_t].contiguous(), Vo_t, wt_t, xa[:, sel_t].contiguous(),
                             xb[:, sel_t].contiguous(), mask=True, R_moon=R, fp64=fp64)
         rx = rel_l2(x.cpu().numpy(), xo)
-        print(f"  fp64={fp64}: forward rel L2 {rv:.2e} on {sel.size} rows, back-projection {rx:.2e}", flush=True)
+        print(f"  fp64={fp64}: forward rel L2 {rv:.2e} on {sel.size} rows, density-compensated image {rx:.2e}",
+              flush=True)
         assert rv <= tol and rx <= tol
 
 

@@ -428,43 +428,48 @@ int viss_batch_autotune(viss_batch* b, const double* I, double* V_scratch, int r
     if (rows > 0 && b->ntiles_g[g] > 0) buckets[bucket_of(b->ntiles_g[g], rows)].push_back((int)g);
   }
   static const int cands[] = {8, 16, 32, 64, 128, 256, 1 << 20};
-  hipEvent_t e0, e1;
-  VISS_HIP(hipEventCreate(&e0));
-  VISS_HIP(hipEventCreate(&e1));
-  WorkItem* tmp = nullptr;
-  size_t tmp_n = 0;
+  // timing events and the candidate work table, released on every return path
+  struct Scratch {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    WorkItem* tmp = nullptr;
+    size_t n = 0;
+    ~Scratch() {
+      if (tmp) (void)hipFree(tmp);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } sc;
+  VISS_HIP(hipEventCreate(&sc.e0));
+  VISS_HIP(hipEventCreate(&sc.e1));
   for (auto& kv : buckets) {
     float best = 1e30f;
     int best_tpc = default_tpc(b->ntiles_g[kv.second[0]]);
     for (int tpc : cands) {
       std::vector<WorkItem> v;
       build_items(b, kv.second, v, b->tune, tpc);
-      if (v.size() > tmp_n) {
-        if (tmp) VISS_HIP(hipFree(tmp));
-        VISS_HIP(hipMalloc((void**)&tmp, v.size() * sizeof(WorkItem)));
-        tmp_n = v.size();
+      if (v.size() > sc.n) {
+        if (sc.tmp) VISS_HIP(hipFree(sc.tmp));
+        sc.tmp = nullptr;
+        VISS_HIP(hipMalloc((void**)&sc.tmp, v.size() * sizeof(WorkItem)));
+        sc.n = v.size();
       }
-      VISS_HIP(hipMemcpyAsync(tmp, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
-      rc = launch_batch(b, tmp, (int64_t)v.size(), V_scratch, s);  // warm
-      if (rc != VISS_OK) break;
-      VISS_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < reps && rc == VISS_OK; ++r) rc = launch_batch(b, tmp, (int64_t)v.size(), V_scratch, s);
-      VISS_HIP(hipEventRecord(e1, s));
-      VISS_HIP(hipEventSynchronize(e1));
+      VISS_HIP(hipMemcpyAsync(sc.tmp, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+      rc = launch_batch(b, sc.tmp, (int64_t)v.size(), V_scratch, s);  // warm
+      if (rc != VISS_OK) return rc;
+      VISS_HIP(hipEventRecord(sc.e0, s));
+      for (int r = 0; r < reps && rc == VISS_OK; ++r) rc = launch_batch(b, sc.tmp, (int64_t)v.size(), V_scratch, s);
+      if (rc != VISS_OK) return rc;
+      VISS_HIP(hipEventRecord(sc.e1, s));
+      VISS_HIP(hipEventSynchronize(sc.e1));
       float ms = 0;
-      VISS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      VISS_HIP(hipEventElapsedTime(&ms, sc.e0, sc.e1));
       if (ms < best) {
         best = ms;
         best_tpc = tpc;
       }
     }
-    if (rc != VISS_OK) break;
     b->tune[kv.first] = best_tpc;
   }
-  if (tmp) (void)hipFree(tmp);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc != VISS_OK) return rc;
   std::vector<int> groups(b->G);
   for (int64_t g = 0; g < b->G; ++g) groups[g] = (int)g;
   std::vector<WorkItem> v;
