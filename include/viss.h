@@ -121,12 +121,21 @@ int viss_weights(const float* bll, const double* w, int64_t K, int32_t nr, int f
  * tiles with bounding cones, packed for the kernels.  Building it costs a
  * sort; viss_forward / viss_backproject build a temporary one per call.  A
  * sky is bound to the device current at creation; calls that share one sky
- * must be ordered (same stream or synchronised). */
+ * must be ordered (same stream or synchronised).
+ *
+ * Creation also pairs every pixel with its antipode (s_b = -s_a to 1e-12),
+ * as HEALPix pixels are: the mixed-precision per-row forward and
+ * back-projection then evaluate one phase and one sin/cos per pair.  Skies
+ * with fewer than half of their pixels paired use the per-pixel kernels.
+ * Creation synchronises the stream once (to size the pair arrays). */
 typedef struct viss_sky viss_sky;
 
 int viss_sky_create(const double* lmn, int64_t P, viss_sky** sky, void* stream);
 int viss_sky_destroy(viss_sky* sky);
 int64_t viss_sky_npix(const viss_sky* sky);
+/* Antipodal pairs and unpaired pixels found at creation (host pointers,
+ * nullable); *uses_pairs = 1 when the mixed per-row kernels use the pairs. */
+int viss_sky_pairs(const viss_sky* sky, int64_t* npair, int64_t* nsingle, int* uses_pairs);
 
 int viss_sky_forward(viss_sky* sky, const double* I, const double* uvw, const double* xyz_a, const double* xyz_b,
                      int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream);

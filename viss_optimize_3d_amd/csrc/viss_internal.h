@@ -44,6 +44,15 @@ struct PixPack {
 static_assert(sizeof(PixPack) == 32, "PixPack is 32 B");
 
 // Per-visibility data read wave-uniformly by the back-projection.
+// An antipodal pixel pair (a, b) with s_b = -s_a, or a single pixel a (Ib = 0).
+// b.s_b = -b.s_a, so one phase and one sin/cos serve both pixels.
+struct PairPack {
+  double l, m, n;  // direction of pixel a (pixel b is at -(l, m, n))
+  float A, D;      // Ia + Ib and Ia - Ib (both pixels seen)
+  float Ia, Ib;    // brightness of a and b (one side seen, or the per-pixel mask)
+};
+static_assert(sizeof(PairPack) == 40, "PairPack is 40 B");
+
 struct VisHot {  // mixed precision: phase in FP64, weighted visibility in FP32
   double u, v, w;
   float yr, yi;
@@ -118,6 +127,16 @@ struct viss_sky {
   float4* cone = nullptr;        // [ntile] bounding cones of kTile-pixel tiles
   float4* bcone = nullptr;       // [nbtile] bounding cones of kBpTile-pixel tiles
   int32_t* tile_valid = nullptr; // [ntile] number of real pixels per tile
+  // Antipodal-pair layout (mixed-precision per-row kernels): pairs sorted along
+  // the Morton curve of pixel a, then the unpaired pixels, each section padded
+  // to whole 64-entry tiles.
+  int64_t npair = 0;             // antipodal pairs found
+  int64_t nsingle = 0;           // pixels without a partner
+  int64_t nptile = 0;            // 64-entry tiles of the pair layout
+  bool use_pairs = false;        // at least half of the pixels are paired
+  viss::PairPack* ppack = nullptr;  // [nptile * 64]
+  int2* pidx = nullptr;             // [nptile * 64] caller indices of (a, b); -1 if absent
+  float4* pcone = nullptr;          // [nptile] bounding cones of the a pixels of each tile
   viss::DevBuf ws;               // per-call visibility packs
 };
 

@@ -54,6 +54,104 @@ __global__ __launch_bounds__(256) void k_forward(const SkyView sv, int64_t ntile
   if (r.act) forward_store(V, k, K, herm, Sr, Si);
 }
 
+// ------------------------------------------------------------ forward, antipodal pairs
+// Mixed precision, skies whose pixels come in antipodal pairs (s_b = -s_a,
+// viss_sky's pair layout).  With theta = 2 pi b.s_a,
+//   I_a e^{-i theta} + I_b e^{+i theta} = (I_a + I_b) cos theta - i (I_a - I_b) sin theta,
+// so one FP64 phase and one sin/cos serve both pixels; the occultation test of
+// b reuses the dot products of a (s_b.r = -s_a.r).  The phase-tracking term c0
+// breaks the symmetry, so it is applied once per row at the end:
+// V = e^{-2 pi i c0} sum_p I_p e^{-2 pi i b.s_p}.
+enum PairMode { kPairBoth = 0, kPairA = 1, kPairB = 2, kPairMixed = 3 };
+
+// Sums over one 64-entry pair tile, in the (Sr, Si) convention of
+// forward_store: Sr = sum I cos, Si = sum I sin of each pixel's own phase.
+template <int MODE>
+__device__ __forceinline__ void pair_tile_forward(const PairPack* __restrict__ tp, double u, double v, double w,
+                                                  const VisMask& mk, float& ar, float& ai) {
+#pragma unroll 4
+  for (int j = 0; j < 64; ++j) {
+    const PairPack q = tp[j];
+    float sn, cs;
+    sincos_rev(rev_fp32(fma(u, q.l, fma(v, q.m, w * q.n))), sn, cs);
+    if (MODE == kPairBoth) {
+      ar = fmaf(q.A, cs, ar);
+      ai = fmaf(q.D, sn, ai);
+    } else if (MODE == kPairA) {
+      ar = fmaf(q.Ia, cs, ar);
+      ai = fmaf(q.Ia, sn, ai);
+    } else if (MODE == kPairB) {
+      ar = fmaf(q.Ib, cs, ar);
+      ai = fmaf(-q.Ib, sn, ai);
+    } else {
+      const double da = fma(q.l, mk.xa, fma(q.m, mk.ya, q.n * mk.za));
+      const double db = fma(q.l, mk.xb, fma(q.m, mk.yb, q.n * mk.zb));
+      const float wa = ((da >= mk.ta) & (db >= mk.tb)) ? q.Ia : 0.f;
+      const float wb = ((-da >= mk.ta) & (-db >= mk.tb)) ? q.Ib : 0.f;
+      ar = fmaf(wa + wb, cs, ar);
+      ai = fmaf(wa - wb, sn, ai);
+    }
+  }
+}
+
+// Tile classification of both sides: the a pixels lie in the cone cn, the b
+// pixels in (-centre, same chord).  Inactive lanes count as seeing nothing and
+// everything (they never force a test).
+__device__ __forceinline__ void classify_pair(const float4 cn, const VisCls& c, bool act, bool& anyA, bool& allA,
+                                              bool& anyB, bool& allB) {
+  const float da = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+  const float db = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, cn.z * c.bz));
+  anyA = act & (da + cn.w >= c.ca - kClsEps) & (db + cn.w >= c.cb - kClsEps);
+  allA = !act | ((da - cn.w >= c.ca + kClsEps) & (db - cn.w >= c.cb + kClsEps));
+  anyB = act & (-da + cn.w >= c.ca - kClsEps) & (-db + cn.w >= c.cb - kClsEps);
+  allB = !act | ((-da - cn.w >= c.ca + kClsEps) & (-db - cn.w >= c.cb + kClsEps));
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_forward_pairs(const PairPack* __restrict__ pp, const float4* __restrict__ pcone,
+                                                       int64_t ntile, int64_t tiles_per_chunk,
+                                                       const double* __restrict__ uvw, const double* __restrict__ xyza,
+                                                       const double* __restrict__ xyzb, int64_t K, double R_moon,
+                                                       double orbit_r, int track, int herm, double* __restrict__ V) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, K, k, k < K, R_moon, orbit_r, track);
+  const int64_t t0 = (int64_t)blockIdx.y * tiles_per_chunk;
+  const int64_t t1 = min(ntile, t0 + tiles_per_chunk);
+  double Sr = 0.0, Si = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    int mode = kPairBoth;
+    if (MASK) {
+      bool anyA, allA, anyB, allB;
+      classify_pair(pcone[t], r.cl, r.act, anyA, allA, anyB, allB);
+      const bool seeA = __ballot(anyA) != 0, seeB = __ballot(anyB) != 0;
+      const bool fullA = __ballot(!allA) == 0, fullB = __ballot(!allB) == 0;
+      if (!seeA && !seeB) continue;
+      mode = (fullA && fullB) ? kPairBoth : (fullA && !seeB) ? kPairA : (!seeA && fullB) ? kPairB : kPairMixed;
+    }
+    const PairPack* __restrict__ tp = pp + t * 64;
+    float ar = 0.f, ai = 0.f;
+    if (mode == kPairBoth)
+      pair_tile_forward<kPairBoth>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    else if (mode == kPairA)
+      pair_tile_forward<kPairA>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    else if (mode == kPairB)
+      pair_tile_forward<kPairB>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    else
+      pair_tile_forward<kPairMixed>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    Sr += (double)ar;
+    Si += (double)ai;
+  }
+  if (!r.act) return;
+  if (r.c0 != 0.0) {  // V e^{-2 pi i c0}: (Sr - i Si)(cos - i sin)
+    double sn, cs;
+    sincospi(2.0 * (r.c0 - rint(r.c0)), &sn, &cs);
+    const double nr = Sr * cs - Si * sn, ni = Sr * sn + Si * cs;
+    Sr = nr;
+    Si = ni;
+  }
+  forward_store(V, k, K, herm, Sr, Si);
+}
+
 // ------------------------------------------------------------ back-projection
 // Prologue: y'_k = wt_k * y_k [* exp(-2 pi i w_k)], packed with u, v, w.
 __global__ void k_bp_prologue(const double* __restrict__ uvw, const double* __restrict__ xyza,
@@ -196,6 +294,93 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
     if (idx[i] >= 0) unsafeAtomicAdd(&x[idx[i]], herm * acc[i]);
 }
 
+// ------------------------------------------------------------ back-projection, antipodal pairs
+// Mixed precision, pair layout: each lane owns one antipodal pair (a, b) of a
+// 64-pair tile.  Re(y e^{+i theta}) = yr cos - yi sin for a and
+// Re(y e^{-i theta}) = yr cos + yi sin for b share the sin/cos.  Rows are
+// classified 64 at a time (lane j tests row j) for both sides of the tile and
+// run in four loops: both sides seen whole, only a, only b, and the rest with
+// the exact per-pixel test.
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __restrict__ pp,
+                                                           const int2* __restrict__ pidx,
+                                                           const float4* __restrict__ pcone, int64_t ntile,
+                                                           const VisHot* __restrict__ hot,
+                                                           const VisMask* __restrict__ vm,
+                                                           const VisCls* __restrict__ vc, int64_t K,
+                                                           int64_t k_per_chunk, double herm, double* __restrict__ x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
+  if (tile >= ntile) return;  // wave-uniform
+  const PairPack q = pp[tile * 64 + lane];
+  const int2 ab = pidx[tile * 64 + lane];
+  const double l = q.l, m = q.m, n = q.n;
+  float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
+  if (MASK) cn = pcone[tile];
+  const int64_t k0 = (int64_t)blockIdx.y * k_per_chunk;
+  const int64_t k1 = min(K, k0 + k_per_chunk);
+  double acc_a = 0.0, acc_b = 0.0;
+  for (int64_t kb = k0; kb < k1; kb += 64) {
+    const int nk = (int)min((int64_t)64, k1 - kb);
+    uint64_t both, onlyA, onlyB, mixed;
+    if (MASK) {
+      bool anyA = false, allA = false, anyB = false, allB = false;
+      if (lane < nk) classify_pair(cn, vc[kb + lane], true, anyA, allA, anyB, allB);
+      const uint64_t sA = __ballot(anyA), fA = __ballot(allA), sB = __ballot(anyB), fB = __ballot(allB);
+      both = fA & fB;
+      onlyA = fA & ~sB;
+      onlyB = fB & ~sA;
+      mixed = (sA | sB) & ~(both | onlyA | onlyB);
+    } else {
+      both = nk == 64 ? ~0ull : ((1ull << nk) - 1ull);
+      onlyA = onlyB = mixed = 0ull;
+    }
+    float sa = 0.f, sb = 0.f;
+    while (both) {
+      const int j = __builtin_ctzll(both);
+      both &= both - 1ull;
+      const VisHot h = hot[kb + j];
+      float sn, cs;
+      sincos_rev(rev_fp32(fma(h.u, l, fma(h.v, m, h.w * n))), sn, cs);
+      sa = fmaf(h.yr, cs, fmaf(-h.yi, sn, sa));
+      sb = fmaf(h.yr, cs, fmaf(h.yi, sn, sb));
+    }
+    while (onlyA) {
+      const int j = __builtin_ctzll(onlyA);
+      onlyA &= onlyA - 1ull;
+      const VisHot h = hot[kb + j];
+      float sn, cs;
+      sincos_rev(rev_fp32(fma(h.u, l, fma(h.v, m, h.w * n))), sn, cs);
+      sa = fmaf(h.yr, cs, fmaf(-h.yi, sn, sa));
+    }
+    while (onlyB) {
+      const int j = __builtin_ctzll(onlyB);
+      onlyB &= onlyB - 1ull;
+      const VisHot h = hot[kb + j];
+      float sn, cs;
+      sincos_rev(rev_fp32(fma(h.u, l, fma(h.v, m, h.w * n))), sn, cs);
+      sb = fmaf(h.yr, cs, fmaf(h.yi, sn, sb));
+    }
+    while (mixed) {
+      const int j = __builtin_ctzll(mixed);
+      mixed &= mixed - 1ull;
+      const VisHot h = hot[kb + j];
+      const VisMask mk = vm[kb + j];
+      float sn, cs;
+      sincos_rev(rev_fp32(fma(h.u, l, fma(h.v, m, h.w * n))), sn, cs);
+      const double da = fma(l, mk.xa, fma(m, mk.ya, n * mk.za));
+      const double db = fma(l, mk.xb, fma(m, mk.yb, n * mk.zb));
+      const float ta = fmaf(h.yr, cs, -h.yi * sn), tb = fmaf(h.yr, cs, h.yi * sn);
+      sa += ((da >= mk.ta) & (db >= mk.tb)) ? ta : 0.f;
+      sb += ((-da >= mk.ta) & (-db >= mk.tb)) ? tb : 0.f;
+    }
+    acc_a += (double)sa;
+    acc_b += (double)sb;
+  }
+  if (ab.x >= 0) unsafeAtomicAdd(&x[ab.x], herm * acc_a);
+  if (ab.y >= 0) unsafeAtomicAdd(&x[ab.y], herm * acc_b);
+}
+
 // ------------------------------------------------------------ visible-pair count
 __global__ __launch_bounds__(256) void k_count_visible(const PixPack* __restrict__ pix,
                                                        const float4* __restrict__ cone,
@@ -273,15 +458,28 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   // 256 tiles x 64 pixels x 32 B = 512 KB per chunk (measured: 191 ms at the
   // precision workload against 242 ms with 4096-tile chunks); smaller chunks
   // when few visibility blocks would otherwise leave CUs idle.
+  // mixed precision over a paired sky: 64-pair tiles, one sin/cos per pair
+  const bool pairs = !f64 && sky->use_pairs;
+  const int64_t ntile = pairs ? sky->nptile : sky->ntile;
   int64_t tpc = env_int("VISS_FWD_TILES", 0);
   if (tpc <= 0) {
     tpc = kFwdTilesPerChunk;
-    while (tpc > 16 && bx * cdiv(sky->ntile, tpc) < kMinBlocks) tpc /= 2;
+    while (tpc > 16 && bx * cdiv(ntile, tpc) < kMinBlocks) tpc /= 2;
   }
-  tpc = std::max(tpc, cdiv(sky->ntile, 65535));  // grid.y limit
+  tpc = std::max(tpc, cdiv(ntile, 65535));  // grid.y limit
   VISS_REQUIRE(bx <= INT32_MAX, "viss_forward: K too large for one launch");
-  dim3 grid((unsigned)bx, (unsigned)cdiv(sky->ntile, tpc));
+  dim3 grid((unsigned)bx, (unsigned)cdiv(ntile, tpc));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
+  if (pairs) {
+    if (mask)
+      k_forward_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, ntile, tpc, uvw, xyza, xyzb, K, R, orb, track,
+                                                 herm, V);
+    else
+      k_forward_pairs<false><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, ntile, tpc, uvw, xyza, xyzb, K, R, orb,
+                                                  track, herm, V);
+    VISS_LAUNCHED(k_forward_pairs);
+    return VISS_OK;
+  }
 #define VISS_FWD(M, F)                                                                                          \
   k_forward<M, F><<<grid, 256, 0, s>>>(sky_view(sky), sky->ntile, tpc, uvw, xyza, xyzb, K, R, orb, \
                                        track, herm, V)
@@ -324,7 +522,9 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
                                               f64 ? 1 : 0, R, orb, hot, hot64, vm, vc);
     VISS_LAUNCHED(k_bp_prologue);
   }
-  const int64_t bx = cdiv(sky->nbtile, 4);
+  // mixed precision over a paired sky: one lane per antipodal pair
+  const bool pairs = !f64 && sky->use_pairs;
+  const int64_t bx = cdiv(pairs ? sky->nptile : sky->nbtile, 4);
   // 4096 rows x 64 B of records per chunk (measured: 187 ms against 210 ms
   // with 82k-row chunks)
   int64_t kpc = env_int("VISS_BP_ROWS", 0);
@@ -338,7 +538,14 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
   const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
 #define VISS_BP(M, F) \
   k_backproject<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, kpc, herm, x)
-  if (mask) {
+  if (pairs) {
+    if (mask)
+      k_backproject_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc, K,
+                                                     kpc, herm, x);
+    else
+      k_backproject_pairs<false><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc,
+                                                      K, kpc, herm, x);
+  } else if (mask) {
     if (f64) VISS_BP(true, true); else VISS_BP(true, false);
   } else {
     if (f64) VISS_BP(false, true); else VISS_BP(false, false);

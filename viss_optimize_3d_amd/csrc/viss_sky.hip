@@ -8,6 +8,7 @@
 // drop the per-pixel test, for a whole wavefront at once.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -163,6 +164,158 @@ __global__ void k_scatter_brightness(PixPack* __restrict__ pix, int64_t P_pad, c
   }
 }
 
+// ------------------------------------------------ antipodal-pair layout
+// HEALPix (and any point-symmetric pixelisation) pairs every pixel with its
+// antipode.  Pairs are found by sorting a 63-bit Morton key of each pixel's
+// canonical direction (its upper-hemisphere representative): a pixel and its
+// antipode share the key, so they end up adjacent.  A run of exactly two equal
+// keys, one pixel from each hemisphere, whose canonical directions agree to
+// 1e-12 is a pair; every other pixel is a single.
+constexpr double kHemiEps = 1e-9;    // |n|, |m| below this count as zero (equator)
+constexpr double kPairTol = 1e-12;   // max |c_a - c_b| of a pair's canonical directions
+
+__device__ inline bool upper_half(double l, double m, double n) {
+  if (n > kHemiEps) return true;
+  if (n < -kHemiEps) return false;
+  if (m > kHemiEps) return true;
+  if (m < -kHemiEps) return false;
+  return l >= 0.0;
+}
+
+__device__ inline uint64_t spread3_21(uint32_t x) {  // 21 bits -> every third bit of 63
+  uint64_t v = x & 0x1fffffu;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+// Nearest of 2^21 levels on [-1, 1]: cell edges fall between dyadic values, so
+// coordinates a pixelisation places exactly (0, +-1/2, ...) and their +-1e-17
+// rounding twins land in the same cell.
+__device__ inline uint32_t quant21(double c) {
+  double q = (c + 1.0) * 1048576.0 + 0.5;
+  q = q < 0.0 ? 0.0 : (q > 2097151.0 ? 2097151.0 : q);
+  return (uint32_t)q;
+}
+
+__device__ inline void canonical(const double* __restrict__ lmn, int64_t P, int64_t i, double& x, double& y,
+                                 double& z, bool& up) {
+  x = lmn[i];
+  y = lmn[P + i];
+  z = lmn[2 * P + i];
+  up = upper_half(x, y, z);
+  if (!up) {
+    x = -x;
+    y = -y;
+    z = -z;
+  }
+}
+
+__global__ void k_pair_keys(const double* __restrict__ lmn, int64_t P, uint64_t* __restrict__ key,
+                            int32_t* __restrict__ val) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    double x, y, z;
+    bool up;
+    canonical(lmn, P, i, x, y, z, up);
+    key[i] = spread3_21(quant21(x)) | (spread3_21(quant21(y)) << 1) | (spread3_21(quant21(z)) << 2);
+    val[i] = (int32_t)i;
+  }
+}
+
+// Sorted position i starts a pair (i, i+1).
+__device__ inline bool pair_starts(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
+                                   const double* __restrict__ lmn, int64_t P, int64_t i) {
+  if (i < 0 || i + 1 >= P) return false;
+  const uint64_t k = key[i];
+  if (key[i + 1] != k || (i > 0 && key[i - 1] == k) || (i + 2 < P && key[i + 2] == k)) return false;
+  double xa, ya, za, xb, yb, zb;
+  bool ua, ub;
+  canonical(lmn, P, val[i], xa, ya, za, ua);
+  canonical(lmn, P, val[i + 1], xb, yb, zb, ub);
+  if (ua == ub) return false;
+  return fabs(xa - xb) <= kPairTol && fabs(ya - yb) <= kPairTol && fabs(za - zb) <= kPairTol;
+}
+
+__global__ void k_pair_flags(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
+                             const double* __restrict__ lmn, int64_t P, uint8_t* __restrict__ is_pair,
+                             uint8_t* __restrict__ is_single) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    const bool st = pair_starts(key, val, lmn, P, i);
+    is_pair[i] = st ? 1 : 0;
+    is_single[i] = (!st && !pair_starts(key, val, lmn, P, i - 1)) ? 1 : 0;
+  }
+}
+
+// Entries [0, npair): pairs; [npair_pad, npair_pad + nsingle): singles; the
+// rest padding (no pixel).
+__global__ void k_pack_pairs(const double* __restrict__ lmn, int64_t P, const int32_t* __restrict__ val,
+                             const int32_t* __restrict__ pair_pos, int64_t npair, int64_t npair_pad,
+                             const int32_t* __restrict__ single_pos, int64_t nsingle, int64_t n,
+                             PairPack* __restrict__ pp, int2* __restrict__ pidx) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    int32_t a = -1, b = -1;
+    if (j < npair) {
+      const int32_t i = pair_pos[j], p0 = val[i], p1 = val[i + 1];
+      const bool up0 = upper_half(lmn[p0], lmn[P + p0], lmn[2 * P + p0]);
+      a = up0 ? p0 : p1;
+      b = up0 ? p1 : p0;
+    } else if (j >= npair_pad && j < npair_pad + nsingle) {
+      a = val[single_pos[j - npair_pad]];
+    }
+    PairPack q;
+    q.l = a >= 0 ? lmn[a] : 0.0;
+    q.m = a >= 0 ? lmn[P + a] : 0.0;
+    q.n = a >= 0 ? lmn[2 * P + a] : 0.0;
+    q.A = q.D = q.Ia = q.Ib = 0.f;
+    pp[j] = q;
+    pidx[j] = make_int2(a, b);
+  }
+}
+
+// One wavefront per 64-entry tile: bounding cone of its a pixels.
+__global__ void k_pair_cones(const PairPack* __restrict__ pp, const int2* __restrict__ pidx, int64_t ntiles,
+                             float4* __restrict__ cone) {
+  const int lane = threadIdx.x & 63;
+  const int64_t t = blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block();
+  if (t >= ntiles) return;
+  const PairPack q = pp[t * 64 + lane];
+  const bool valid = pidx[t * 64 + lane].x >= 0;
+  double sx = valid ? q.l : 0.0, sy = valid ? q.m : 0.0, sz = valid ? q.n : 0.0;
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  const int nvalid = wave_isum(valid ? 1 : 0);
+  const double nrm = sqrt(sx * sx + sy * sy + sz * sz);
+  double cx = 0, cy = 0, cz = 1, chord = 2.0;
+  if (nrm > 1e-6 * nvalid && nvalid > 0) {
+    cx = sx / nrm;
+    cy = sy / nrm;
+    cz = sz / nrm;
+    double mx = 0;
+    if (valid) {
+      const double dx = q.l - cx, dy = q.m - cy, dz = q.n - cz;
+      mx = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    chord = wave_max(mx) * (1.0 + 1e-6) + 1e-6;
+  }
+  if (lane == 0) cone[t] = make_float4((float)cx, (float)cy, (float)cz, (float)chord);
+}
+
+__global__ void k_scatter_pairs(PairPack* __restrict__ pp, const int2* __restrict__ pidx, int64_t n,
+                                const double* __restrict__ I) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int2 ab = pidx[j];
+    const double ia = ab.x >= 0 ? I[ab.x] : 0.0, ib = ab.y >= 0 ? I[ab.y] : 0.0;
+    pp[j].A = (float)(ia + ib);
+    pp[j].D = (float)(ia - ib);
+    pp[j].Ia = (float)ia;
+    pp[j].Ib = (float)ib;
+  }
+}
+
 static unsigned grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
   if (g > 65536) g = 65536;
@@ -178,6 +331,77 @@ int check_device(const viss_sky* sky) {
     return VISS_ERR_INVALID;
   }
   return VISS_OK;
+}
+
+// Antipodal-pair layout (see k_pair_keys).  Synchronises once, to size the
+// arrays from the pair count.
+static int sky_build_pairs(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
+  uint64_t *k_in = nullptr, *k_out = nullptr;
+  int32_t *v_in = nullptr, *v_out = nullptr, *pair_pos = nullptr, *single_pos = nullptr, *counts = nullptr;
+  uint8_t *is_pair = nullptr, *is_single = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0, sel_bytes = 0;
+  VISS_HIP(hipMallocAsync((void**)&k_in, sizeof(uint64_t) * P, s));
+  VISS_HIP(hipMallocAsync((void**)&k_out, sizeof(uint64_t) * P, s));
+  VISS_HIP(hipMallocAsync((void**)&v_in, sizeof(int32_t) * P, s));
+  VISS_HIP(hipMallocAsync((void**)&v_out, sizeof(int32_t) * P, s));
+  VISS_HIP(hipMallocAsync((void**)&pair_pos, sizeof(int32_t) * P, s));
+  VISS_HIP(hipMallocAsync((void**)&single_pos, sizeof(int32_t) * P, s));
+  VISS_HIP(hipMallocAsync((void**)&counts, sizeof(int32_t) * 2, s));
+  VISS_HIP(hipMallocAsync((void**)&is_pair, P, s));
+  VISS_HIP(hipMallocAsync((void**)&is_single, P, s));
+  k_pair_keys<<<grid_for(P, 256), 256, 0, s>>>(lmn, P, k_in, v_in);
+  VISS_LAUNCHED(k_pair_keys);
+  VISS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k_in, k_out, v_in, v_out, (int)P, 0, 63, s));
+  hipcub::CountingInputIterator<int32_t> iota(0);
+  VISS_HIP(hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, iota, is_pair, pair_pos, counts, (int)P, s));
+  tmp_bytes = std::max(tmp_bytes, sel_bytes);
+  VISS_HIP(hipMallocAsync(&tmp, tmp_bytes, s));
+  VISS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k_in, k_out, v_in, v_out, (int)P, 0, 63, s));
+  k_pair_flags<<<grid_for(P, 256), 256, 0, s>>>(k_out, v_out, lmn, P, is_pair, is_single);
+  VISS_LAUNCHED(k_pair_flags);
+  size_t b = tmp_bytes;
+  VISS_HIP(hipcub::DeviceSelect::Flagged(tmp, b, iota, is_pair, pair_pos, counts, (int)P, s));
+  b = tmp_bytes;
+  VISS_HIP(hipcub::DeviceSelect::Flagged(tmp, b, iota, is_single, single_pos, counts + 1, (int)P, s));
+  int32_t hc[2] = {0, 0};
+  VISS_HIP(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
+  VISS_HIP(hipStreamSynchronize(s));
+  sky->npair = hc[0];
+  sky->nsingle = hc[1];
+  int rc = VISS_OK;
+  if (2 * sky->npair + sky->nsingle != P) {
+    set_error("viss_sky: pair detection lost pixels (%lld pairs, %lld singles, %lld pixels)",
+              (long long)sky->npair, (long long)sky->nsingle, (long long)P);
+    rc = VISS_ERR_HIP;
+  } else {
+    const int64_t npair_pad = (sky->npair + 63) / 64 * 64, nsingle_pad = (sky->nsingle + 63) / 64 * 64;
+    sky->nptile = (npair_pad + nsingle_pad) / 64;
+    sky->use_pairs = 2 * sky->npair >= P / 2;
+    const int64_t n = sky->nptile * 64;
+    if (n > 0) {
+      VISS_HIP(hipMalloc((void**)&sky->ppack, sizeof(PairPack) * n));
+      VISS_HIP(hipMalloc((void**)&sky->pidx, sizeof(int2) * n));
+      VISS_HIP(hipMalloc((void**)&sky->pcone, sizeof(float4) * sky->nptile));
+      k_pack_pairs<<<grid_for(n, 256), 256, 0, s>>>(lmn, P, v_out, pair_pos, sky->npair, npair_pad, single_pos,
+                                                     sky->nsingle, n, sky->ppack, sky->pidx);
+      VISS_LAUNCHED(k_pack_pairs);
+      k_pair_cones<<<(unsigned)((sky->nptile + 3) / 4), 256, 0, s>>>(sky->ppack, sky->pidx, sky->nptile,
+                                                                      sky->pcone);
+      VISS_LAUNCHED(k_pair_cones);
+    }
+  }
+  VISS_HIP(hipFreeAsync(tmp, s));
+  VISS_HIP(hipFreeAsync(k_in, s));
+  VISS_HIP(hipFreeAsync(k_out, s));
+  VISS_HIP(hipFreeAsync(v_in, s));
+  VISS_HIP(hipFreeAsync(v_out, s));
+  VISS_HIP(hipFreeAsync(pair_pos, s));
+  VISS_HIP(hipFreeAsync(single_pos, s));
+  VISS_HIP(hipFreeAsync(counts, s));
+  VISS_HIP(hipFreeAsync(is_pair, s));
+  VISS_HIP(hipFreeAsync(is_single, s));
+  return rc;
 }
 
 int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
@@ -221,7 +445,7 @@ int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
   VISS_HIP(hipFreeAsync(k_out, s));
   VISS_HIP(hipFreeAsync(v_in, s));
   VISS_HIP(hipFreeAsync(v_out, s));
-  return VISS_OK;
+  return sky_build_pairs(sky, lmn, P, s);
 }
 
 void sky_free(viss_sky* sky) {
@@ -231,6 +455,12 @@ void sky_free(viss_sky* sky) {
   if (sky->cone) (void)hipFree(sky->cone);
   if (sky->bcone) (void)hipFree(sky->bcone);
   if (sky->tile_valid) (void)hipFree(sky->tile_valid);
+  if (sky->ppack) (void)hipFree(sky->ppack);
+  if (sky->pidx) (void)hipFree(sky->pidx);
+  if (sky->pcone) (void)hipFree(sky->pcone);
+  sky->ppack = nullptr;
+  sky->pidx = nullptr;
+  sky->pcone = nullptr;
   sky->ws.release();
   sky->pix = nullptr;
   sky->I64 = nullptr;
@@ -241,6 +471,10 @@ void sky_free(viss_sky* sky) {
 int sky_set_brightness(viss_sky* sky, const double* I, bool want64, hipStream_t s) {
   k_scatter_brightness<<<grid_for(sky->P_pad, 256), 256, 0, s>>>(sky->pix, sky->P_pad, I, sky->I64, want64 ? 1 : 0);
   VISS_LAUNCHED(k_scatter_brightness);
+  if (sky->nptile > 0) {
+    k_scatter_pairs<<<grid_for(sky->nptile * 64, 256), 256, 0, s>>>(sky->ppack, sky->pidx, sky->nptile * 64, I);
+    VISS_LAUNCHED(k_scatter_pairs);
+  }
   return VISS_OK;
 }
 
@@ -292,5 +526,13 @@ int viss_sky_destroy(viss_sky* sky) {
 }
 
 int64_t viss_sky_npix(const viss_sky* sky) { return sky ? sky->P : -1; }
+
+int viss_sky_pairs(const viss_sky* sky, int64_t* npair, int64_t* nsingle, int* uses_pairs) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_pairs: sky is NULL");
+  if (npair) *npair = sky->npair;
+  if (nsingle) *nsingle = sky->nsingle;
+  if (uses_pairs) *uses_pairs = sky->use_pairs ? 1 : 0;
+  return VISS_OK;
+}
 
 }  // extern "C"

@@ -98,6 +98,14 @@ class Sky:
             check("viss_sky_create", L.viss_sky_create(_p(self.lmn), self.P, ctypes.byref(h), _stream()))
         self._h = h
 
+    def pairs(self):
+        """Antipodal pairs and unpaired pixels found at creation, and whether the
+        mixed per-row kernels use the pairs."""
+        n_pair, n_single, uses = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        check("viss_sky_pairs", _lib.load().viss_sky_pairs(self._h, ctypes.byref(n_pair), ctypes.byref(n_single),
+                                                           ctypes.byref(uses)))
+        return {"pairs": n_pair.value, "singles": n_single.value, "uses_pairs": bool(uses.value)}
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             torch.cuda.synchronize(self.device)
