@@ -2,9 +2,12 @@
 with uneven compacted tile lists in one launch, eager and hipGraph-replayed,
 against per-group launches and the FP64 oracle.
 
-Tolerances: batched vs per-group launches of the same kernels within 1e-12
-relative L2 (same per-tile FP32 partials; only the FP64 combine order
-differs); against the FP64 oracle within 1e-5 (mixed) / 1e-11 (FP64).
+Tolerances: batched vs per-group launches within 1e-12 relative L2 for the
+all-FP64 kernels (the same kernel on the same tiles; only the FP64 combine
+order differs) and 1e-6 for mixed precision (per-group mixed launches on a
+paired sky use the antipodal-pair kernel, the batch the per-pixel one: two
+FP32 formulations); eager vs graph-replayed batches within 1e-12; against the
+FP64 oracle within 1e-5 (mixed) / 1e-11 (FP64).
 """
 import numpy as np
 import pytest
@@ -42,7 +45,7 @@ def test_batch_matches_per_group_and_oracle(setup, fp64):
     parts = [sky.forward(I, uvw[:, offs[g]:offs[g + 1]], xa[:, offs[g]:offs[g + 1]], xb[:, offs[g]:offs[g + 1]],
                          mask=True, R_moon=R, fp64=fp64) for g in range(len(offs) - 1)]
     Vg = torch.cat(parts)
-    assert rel_l2(V.cpu().numpy(), Vg.cpu().numpy()) < 1e-12
+    assert rel_l2(V.cpu().numpy(), Vg.cpu().numpy()) < (1e-12 if fp64 else 1e-6)
     ln = lmn.cpu().numpy()
     rows = np.r_[0:200, offs[-2]:offs[-1]]
     ref = oracle.forward(*ln, I.cpu().numpy(), host[0][:, rows], host[1][:, rows], host[2][:, rows],

@@ -174,12 +174,12 @@ __global__ void k_scatter_brightness(PixPack* __restrict__ pix, int64_t P_pad, c
 constexpr double kHemiEps = 1e-9;    // |n|, |m| below this count as zero (equator)
 constexpr double kPairTol = 1e-12;   // max |c_a - c_b| of a pair's canonical directions
 
+// Written with selects only: the same test as nested early returns plus a
+// conditional negation in canonical() compiled (hipcc, gfx950) to code that
+// dropped the negation for lanes with |n| <= kHemiEps.
 __device__ inline bool upper_half(double l, double m, double n) {
-  if (n > kHemiEps) return true;
-  if (n < -kHemiEps) return false;
-  if (m > kHemiEps) return true;
-  if (m < -kHemiEps) return false;
-  return l >= 0.0;
+  const bool by_n = fabs(n) > kHemiEps, by_m = fabs(m) > kHemiEps;
+  return by_n ? (n > 0.0) : (by_m ? (m > 0.0) : (l >= 0.0));
 }
 
 __device__ inline uint64_t spread3_21(uint32_t x) {  // 21 bits -> every third bit of 63
@@ -203,15 +203,12 @@ __device__ inline uint32_t quant21(double c) {
 
 __device__ inline void canonical(const double* __restrict__ lmn, int64_t P, int64_t i, double& x, double& y,
                                  double& z, bool& up) {
-  x = lmn[i];
-  y = lmn[P + i];
-  z = lmn[2 * P + i];
-  up = upper_half(x, y, z);
-  if (!up) {
-    x = -x;
-    y = -y;
-    z = -z;
-  }
+  const double l = lmn[i], m = lmn[P + i], n = lmn[2 * P + i];
+  up = upper_half(l, m, n);
+  const double sg = up ? 1.0 : -1.0;
+  x = sg * l;
+  y = sg * m;
+  z = sg * n;
 }
 
 __global__ void k_pair_keys(const double* __restrict__ lmn, int64_t P, uint64_t* __restrict__ key,
@@ -229,14 +226,16 @@ __global__ void k_pair_keys(const double* __restrict__ lmn, int64_t P, uint64_t*
 __device__ inline bool pair_starts(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
                                    const double* __restrict__ lmn, int64_t P, int64_t i) {
   if (i < 0 || i + 1 >= P) return false;
+  // every condition evaluated and combined with &, no early returns (see upper_half)
   const uint64_t k = key[i];
-  if (key[i + 1] != k || (i > 0 && key[i - 1] == k) || (i + 2 < P && key[i + 2] == k)) return false;
+  const bool before = i > 0 ? key[i - 1] == k : false;
+  const bool after = i + 2 < P ? key[i + 2] == k : false;
   double xa, ya, za, xb, yb, zb;
   bool ua, ub;
   canonical(lmn, P, val[i], xa, ya, za, ua);
   canonical(lmn, P, val[i + 1], xb, yb, zb, ub);
-  if (ua == ub) return false;
-  return fabs(xa - xb) <= kPairTol && fabs(ya - yb) <= kPairTol && fabs(za - zb) <= kPairTol;
+  return (key[i + 1] == k) & !before & !after & (ua != ub) & (fabs(xa - xb) <= kPairTol) &
+         (fabs(ya - yb) <= kPairTol) & (fabs(za - zb) <= kPairTol);
 }
 
 __global__ void k_pair_flags(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
