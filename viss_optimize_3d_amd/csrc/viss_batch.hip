@@ -103,7 +103,9 @@ __global__ __launch_bounds__(256) void k_group_bounds(const double* __restrict__
 }
 
 // Compacted list of tiles some satellite of the group may see.  One block per group.
-__global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ cone, int64_t ntile,
+// both_sides: tiles of antipodal pairs, kept when the a pixels (cone c) or the
+// b pixels (cone -c) may be seen.
+__global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ cone, int64_t ntile, int both_sides,
                                                      const float4* __restrict__ gcone, const float* __restrict__ gcut,
                                                      int32_t* __restrict__ tiles, int32_t* __restrict__ count) {
   __shared__ int wcount[4];
@@ -119,6 +121,7 @@ __global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ 
       const float4 c = cone[t];
       const float d = fmaf(c.x, gc.x, fmaf(c.y, gc.y, c.z * gc.z));
       keep = d + c.w + gc.w >= cut - kClsEps;
+      if (both_sides) keep |= -d + c.w + gc.w >= cut - kClsEps;
     }
     const uint64_t bal = __ballot(keep);
     if (lane == 0) wcount[wv] = __popcll(bal);
@@ -158,6 +161,29 @@ __global__ __launch_bounds__(256) void k_forward_batch(const SkyView sv, const i
   if (r.act) forward_store(V, k, Ktot, 0, Sr, Si);
 }
 
+// The same work table over a paired sky (mixed precision): items are 64 rows x
+// a range of the group's list of 64-pair tiles.
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_forward_batch_pairs(const PairPack* __restrict__ pp,
+                                                             const float4* __restrict__ pcone,
+                                                             const int32_t* __restrict__ tiles, int64_t ntile,
+                                                             const WorkItem* __restrict__ items, int64_t n_items,
+                                                             const double* __restrict__ uvw,
+                                                             const double* __restrict__ xyza,
+                                                             const double* __restrict__ xyzb, int64_t Ktot,
+                                                             double R_moon, double orbit_r, int track,
+                                                             double* __restrict__ V) {
+  const int lane = threadIdx.x & 63;
+  const int64_t it = (int64_t)blockIdx.x * 4 + wave_in_block();
+  if (it >= n_items) return;  // wave-uniform
+  const WorkItem w = items[it];
+  const int64_t k = w.row0 + lane;
+  const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, lane < w.nrows, R_moon, orbit_r, track);
+  double Sr = 0.0, Si = 0.0;
+  forward_pair_tiles<MASK>(pp, pcone, tiles + (int64_t)w.g * ntile, w.j0, w.j1, r, Sr, Si);
+  if (r.act) forward_pair_store(V, k, Ktot, 0, r.c0, Sr, Si);
+}
+
 }  // namespace viss
 
 using namespace viss;
@@ -170,6 +196,8 @@ struct viss_batch {
   const double *uvw = nullptr, *xa = nullptr, *xb = nullptr;
   double R = 0, orb = 0;
   int flags = 0;
+  bool pairs = false;   // mixed precision over a paired sky: tiles of 64 antipodal pairs
+  int64_t ntile = 0;    // tiles of the layout in use (pairs: sky->nptile, else sky->ntile)
   int64_t* d_offs = nullptr;
   float4* gcone = nullptr;
   float* gcut = nullptr;
@@ -262,8 +290,18 @@ static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, d
   const unsigned grid = (unsigned)blocks;
   const int track = (b->flags & VISS_PHASE_TRACK) ? 1 : 0;
   const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
+  if (b->pairs) {
+    if (mask)
+      k_forward_batch_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, b->tiles, b->ntile, items, n_items,
+                                                       b->uvw, b->xa, b->xb, b->Ktot, b->R, b->orb, track, V);
+    else
+      k_forward_batch_pairs<false><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, b->tiles, b->ntile, items, n_items,
+                                                        b->uvw, b->xa, b->xb, b->Ktot, b->R, b->orb, track, V);
+    VISS_LAUNCHED(k_forward_batch_pairs);
+    return VISS_OK;
+  }
 #define VISS_FB(M, F)                                                                                             \
-  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky_view(sky), b->tiles, sky->ntile, items, n_items, b->uvw, b->xa, \
+  k_forward_batch<M, F><<<grid, 256, 0, s>>>(sky_view(sky), b->tiles, b->ntile, items, n_items, b->uvw, b->xa, \
                                              b->xb, b->Ktot, b->R, b->orb, track, V)
   if (mask) {
     if (f64) VISS_FB(true, true); else VISS_FB(true, false);
@@ -292,22 +330,25 @@ static int batch_build(viss_batch* b, hipStream_t s) {
   VISS_HIP(hipMalloc((void**)&b->d_offs, sizeof(int64_t) * (G + 1)));
   VISS_HIP(hipMalloc((void**)&b->gcone, sizeof(float4) * G));
   VISS_HIP(hipMalloc((void**)&b->gcut, sizeof(float) * G));
-  VISS_HIP(hipMalloc((void**)&b->tiles, sizeof(int32_t) * G * sky->ntile));
+  b->pairs = !(b->flags & VISS_PREC_FP64) && sky->use_pairs;
+  b->ntile = b->pairs ? sky->nptile : sky->ntile;
+  VISS_HIP(hipMalloc((void**)&b->tiles, sizeof(int32_t) * G * b->ntile));
   VISS_HIP(hipMalloc((void**)&b->d_count, sizeof(int32_t) * G));
   VISS_HIP(hipMemcpyAsync(b->d_offs, b->offs.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, s));
-  b->ntiles_g.assign(G, (int32_t)sky->ntile);
+  b->ntiles_g.assign(G, (int32_t)b->ntile);
   std::vector<int32_t> all;
   if (b->flags & VISS_MASK) {
     k_group_bounds<<<(unsigned)G, 256, 0, s>>>(b->xa, b->xb, b->Ktot, b->d_offs, b->R, b->orb, b->gcone, b->gcut);
     VISS_LAUNCHED(k_group_bounds);
-    k_group_tiles<<<(unsigned)G, 256, 0, s>>>(sky->cone, sky->ntile, b->gcone, b->gcut, b->tiles, b->d_count);
+    k_group_tiles<<<(unsigned)G, 256, 0, s>>>(b->pairs ? sky->pcone : sky->cone, b->ntile, b->pairs ? 1 : 0, b->gcone,
+                                              b->gcut, b->tiles, b->d_count);
     VISS_LAUNCHED(k_group_tiles);
     VISS_HIP(hipMemcpyAsync(b->ntiles_g.data(), b->d_count, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
   } else {
-    all.resize(sky->ntile);
-    for (int64_t t = 0; t < sky->ntile; ++t) all[t] = (int32_t)t;
+    all.resize(b->ntile);
+    for (int64_t t = 0; t < b->ntile; ++t) all[t] = (int32_t)t;
     for (int64_t g = 0; g < G; ++g)
-      VISS_HIP(hipMemcpyAsync(b->tiles + g * sky->ntile, all.data(), sizeof(int32_t) * sky->ntile,
+      VISS_HIP(hipMemcpyAsync(b->tiles + g * b->ntile, all.data(), sizeof(int32_t) * b->ntile,
                               hipMemcpyHostToDevice, s));
   }
   VISS_HIP(hipStreamSynchronize(s));
@@ -514,7 +555,7 @@ int viss_batch_stats(const viss_batch* b, int64_t* visible_tiles, int64_t* total
   int64_t vt = 0;
   for (int32_t c : b->ntiles_g) vt += c;
   if (visible_tiles) *visible_tiles = vt;
-  if (total_tiles) *total_tiles = b->G * b->sky->ntile;
+  if (total_tiles) *total_tiles = b->G * b->ntile;
   if (n_items) *n_items = b->n_items;
   if (grid_blocks) *grid_blocks = (b->n_items + 3) / 4;
   return VISS_OK;

@@ -162,4 +162,104 @@ __device__ __forceinline__ void forward_store(double* __restrict__ V, int64_t k,
   }
 }
 
+// ------------------------------------------------------------ forward, antipodal pairs
+// Mixed precision, skies whose pixels come in antipodal pairs (s_b = -s_a,
+// viss_sky's pair layout).  With theta = 2 pi b.s_a,
+//   I_a e^{-i theta} + I_b e^{+i theta} = (I_a + I_b) cos theta - i (I_a - I_b) sin theta,
+// so one FP64 phase and one sin/cos serve both pixels; the occultation test of
+// b reuses the dot products of a (s_b.r = -s_a.r).  The phase-tracking term c0
+// breaks the symmetry, so it is applied once per row at the end:
+// V = e^{-2 pi i c0} sum_p I_p e^{-2 pi i b.s_p}.
+enum PairMode { kPairBoth = 0, kPairA = 1, kPairB = 2, kPairMixed = 3 };
+
+// Sums over one 64-entry pair tile, in the (Sr, Si) convention of
+// forward_store: Sr = sum I cos, Si = sum I sin of each pixel's own phase.
+template <int MODE>
+__device__ __forceinline__ void pair_tile_forward(const PairPack* __restrict__ tp, double u, double v, double w,
+                                                  const VisMask& mk, float& ar, float& ai) {
+#pragma unroll 4
+  for (int j = 0; j < 64; ++j) {
+    const PairPack q = tp[j];
+    float sn, cs;
+    sincos_rev(rev_fp32(fma(u, q.l, fma(v, q.m, w * q.n))), sn, cs);
+    if (MODE == kPairBoth) {
+      ar = fmaf(q.A, cs, ar);
+      ai = fmaf(q.D, sn, ai);
+    } else if (MODE == kPairA) {
+      ar = fmaf(q.Ia, cs, ar);
+      ai = fmaf(q.Ia, sn, ai);
+    } else if (MODE == kPairB) {
+      ar = fmaf(q.Ib, cs, ar);
+      ai = fmaf(-q.Ib, sn, ai);
+    } else {
+      const double da = fma(q.l, mk.xa, fma(q.m, mk.ya, q.n * mk.za));
+      const double db = fma(q.l, mk.xb, fma(q.m, mk.yb, q.n * mk.zb));
+      const float wa = ((da >= mk.ta) & (db >= mk.tb)) ? q.Ia : 0.f;
+      const float wb = ((-da >= mk.ta) & (-db >= mk.tb)) ? q.Ib : 0.f;
+      ar = fmaf(wa + wb, cs, ar);
+      ai = fmaf(wa - wb, sn, ai);
+    }
+  }
+}
+
+// Tile classification of both sides: the a pixels lie in the cone cn, the b
+// pixels in (-centre, same chord).  Inactive lanes count as seeing nothing and
+// everything (they never force a test).
+__device__ __forceinline__ void classify_pair(const float4 cn, const VisCls& c, bool act, bool& anyA, bool& allA,
+                                              bool& anyB, bool& allB) {
+  const float da = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
+  const float db = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, cn.z * c.bz));
+  anyA = act & (da + cn.w >= c.ca - kClsEps) & (db + cn.w >= c.cb - kClsEps);
+  allA = !act | ((da - cn.w >= c.ca + kClsEps) & (db - cn.w >= c.cb + kClsEps));
+  anyB = act & (-da + cn.w >= c.ca - kClsEps) & (-db + cn.w >= c.cb - kClsEps);
+  allB = !act | ((-da - cn.w >= c.ca + kClsEps) & (-db - cn.w >= c.cb + kClsEps));
+}
+
+// Accumulates the pair tiles t = list[j] (t = j when list is NULL), j in
+// [j0, j1), into (Sr, Si) without the phase-tracking term; all lanes of a
+// wavefront walk the same tiles.
+template <bool MASK>
+__device__ __forceinline__ void forward_pair_tiles(const PairPack* __restrict__ pp, const float4* __restrict__ pcone,
+                                                   const int32_t* __restrict__ list, int64_t j0, int64_t j1,
+                                                   const FwdRow& r, double& Sr, double& Si) {
+  for (int64_t jt = j0; jt < j1; ++jt) {
+    const int64_t t = list ? (int64_t)list[jt] : jt;
+    int mode = kPairBoth;
+    if (MASK) {
+      bool anyA, allA, anyB, allB;
+      classify_pair(pcone[t], r.cl, r.act, anyA, allA, anyB, allB);
+      const bool seeA = __ballot(anyA) != 0, seeB = __ballot(anyB) != 0;
+      const bool fullA = __ballot(!allA) == 0, fullB = __ballot(!allB) == 0;
+      if (!seeA && !seeB) continue;
+      mode = (fullA && fullB) ? kPairBoth : (fullA && !seeB) ? kPairA : (!seeA && fullB) ? kPairB : kPairMixed;
+    }
+    const PairPack* __restrict__ tp = pp + t * 64;
+    float ar = 0.f, ai = 0.f;
+    if (mode == kPairBoth)
+      pair_tile_forward<kPairBoth>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    else if (mode == kPairA)
+      pair_tile_forward<kPairA>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    else if (mode == kPairB)
+      pair_tile_forward<kPairB>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    else
+      pair_tile_forward<kPairMixed>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    Sr += (double)ar;
+    Si += (double)ai;
+  }
+}
+
+// Applies the phase-tracking term, V e^{-2 pi i c0} = (Sr - i Si)(cos - i sin),
+// and stores as forward_store.
+__device__ __forceinline__ void forward_pair_store(double* __restrict__ V, int64_t k, int64_t K, int herm, double c0,
+                                                   double Sr, double Si) {
+  if (c0 != 0.0) {
+    double sn, cs;
+    sincospi(2.0 * (c0 - rint(c0)), &sn, &cs);
+    const double nr = Sr * cs - Si * sn, ni = Sr * sn + Si * cs;
+    Sr = nr;
+    Si = ni;
+  }
+  forward_store(V, k, K, herm, Sr, Si);
+}
+
 }  // namespace viss
