@@ -170,11 +170,14 @@ __device__ __forceinline__ void forward_store(double* __restrict__ V, int64_t k,
 // b reuses the dot products of a (s_b.r = -s_a.r).  The phase-tracking term c0
 // breaks the symmetry, so it is applied once per row at the end:
 // V = e^{-2 pi i c0} sum_p I_p e^{-2 pi i b.s_p}.
-enum PairMode { kPairBoth = 0, kPairA = 1, kPairB = 2, kPairMixed = 3 };
+// State of one side of a pair tile for every lane of the wavefront.
+enum SideMode { kSideNone = 0, kSideFull = 1, kSideTest = 2 };
 
 // Sums over one 64-entry pair tile, in the (Sr, Si) convention of
 // forward_store: Sr = sum I cos, Si = sum I sin of each pixel's own phase.
-template <int MODE>
+// TA / TB: side a / b seen by no lane, wholly by every lane, or tested per
+// pixel (the exact FP64 occultation test; b reuses a's dot products negated).
+template <int TA, int TB>
 __device__ __forceinline__ void pair_tile_forward(const PairPack* __restrict__ tp, double u, double v, double w,
                                                   const VisMask& mk, float& ar, float& ai) {
 #pragma unroll 4
@@ -182,22 +185,31 @@ __device__ __forceinline__ void pair_tile_forward(const PairPack* __restrict__ t
     const PairPack q = tp[j];
     float sn, cs;
     sincos_rev(rev_fp32(fma(u, q.l, fma(v, q.m, w * q.n))), sn, cs);
-    if (MODE == kPairBoth) {
+    if (TA == kSideFull && TB == kSideFull) {
       ar = fmaf(q.A, cs, ar);
       ai = fmaf(q.D, sn, ai);
-    } else if (MODE == kPairA) {
-      ar = fmaf(q.Ia, cs, ar);
-      ai = fmaf(q.Ia, sn, ai);
-    } else if (MODE == kPairB) {
-      ar = fmaf(q.Ib, cs, ar);
-      ai = fmaf(-q.Ib, sn, ai);
     } else {
-      const double da = fma(q.l, mk.xa, fma(q.m, mk.ya, q.n * mk.za));
-      const double db = fma(q.l, mk.xb, fma(q.m, mk.yb, q.n * mk.zb));
-      const float wa = ((da >= mk.ta) & (db >= mk.tb)) ? q.Ia : 0.f;
-      const float wb = ((-da >= mk.ta) & (-db >= mk.tb)) ? q.Ib : 0.f;
-      ar = fmaf(wa + wb, cs, ar);
-      ai = fmaf(wa - wb, sn, ai);
+      double da = 0.0, db = 0.0;
+      if (TA == kSideTest || TB == kSideTest) {
+        da = fma(q.l, mk.xa, fma(q.m, mk.ya, q.n * mk.za));
+        db = fma(q.l, mk.xb, fma(q.m, mk.yb, q.n * mk.zb));
+      }
+      const float wa = TA == kSideNone ? 0.f
+                       : TA == kSideFull ? q.Ia
+                                         : (((da >= mk.ta) & (db >= mk.tb)) ? q.Ia : 0.f);
+      const float wb = TB == kSideNone ? 0.f
+                       : TB == kSideFull ? q.Ib
+                                         : (((-da >= mk.ta) & (-db >= mk.tb)) ? q.Ib : 0.f);
+      if (TA != kSideNone && TB != kSideNone) {
+        ar = fmaf(wa + wb, cs, ar);
+        ai = fmaf(wa - wb, sn, ai);
+      } else if (TA != kSideNone) {
+        ar = fmaf(wa, cs, ar);
+        ai = fmaf(wa, sn, ai);
+      } else {
+        ar = fmaf(wb, cs, ar);
+        ai = fmaf(-wb, sn, ai);
+      }
     }
   }
 }
@@ -224,25 +236,27 @@ __device__ __forceinline__ void forward_pair_tiles(const PairPack* __restrict__ 
                                                    const FwdRow& r, double& Sr, double& Si) {
   for (int64_t jt = j0; jt < j1; ++jt) {
     const int64_t t = list ? (int64_t)list[jt] : jt;
-    int mode = kPairBoth;
+    int ma = kSideFull, mb = kSideFull;
     if (MASK) {
       bool anyA, allA, anyB, allB;
       classify_pair(pcone[t], r.cl, r.act, anyA, allA, anyB, allB);
       const bool seeA = __ballot(anyA) != 0, seeB = __ballot(anyB) != 0;
-      const bool fullA = __ballot(!allA) == 0, fullB = __ballot(!allB) == 0;
       if (!seeA && !seeB) continue;
-      mode = (fullA && fullB) ? kPairBoth : (fullA && !seeB) ? kPairA : (!seeA && fullB) ? kPairB : kPairMixed;
+      ma = !seeA ? kSideNone : (__ballot(!allA) == 0 ? kSideFull : kSideTest);
+      mb = !seeB ? kSideNone : (__ballot(!allB) == 0 ? kSideFull : kSideTest);
     }
     const PairPack* __restrict__ tp = pp + t * 64;
     float ar = 0.f, ai = 0.f;
-    if (mode == kPairBoth)
-      pair_tile_forward<kPairBoth>(tp, r.u, r.v, r.w, r.mk, ar, ai);
-    else if (mode == kPairA)
-      pair_tile_forward<kPairA>(tp, r.u, r.v, r.w, r.mk, ar, ai);
-    else if (mode == kPairB)
-      pair_tile_forward<kPairB>(tp, r.u, r.v, r.w, r.mk, ar, ai);
-    else
-      pair_tile_forward<kPairMixed>(tp, r.u, r.v, r.w, r.mk, ar, ai);
+    switch (ma * 3 + mb) {  // wave-uniform
+      case kSideFull * 3 + kSideFull: pair_tile_forward<kSideFull, kSideFull>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      case kSideFull * 3 + kSideNone: pair_tile_forward<kSideFull, kSideNone>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      case kSideNone * 3 + kSideFull: pair_tile_forward<kSideNone, kSideFull>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      case kSideFull * 3 + kSideTest: pair_tile_forward<kSideFull, kSideTest>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      case kSideTest * 3 + kSideFull: pair_tile_forward<kSideTest, kSideFull>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      case kSideTest * 3 + kSideNone: pair_tile_forward<kSideTest, kSideNone>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      case kSideNone * 3 + kSideTest: pair_tile_forward<kSideNone, kSideTest>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+      default: pair_tile_forward<kSideTest, kSideTest>(tp, r.u, r.v, r.w, r.mk, ar, ai); break;
+    }
     Sr += (double)ar;
     Si += (double)ai;
   }
