@@ -97,3 +97,34 @@ def test_pair_layouts_against_oracle(ops, kind):
         print(f"{kind} mask={mask} track={track} herm={herm}: forward {rv:.2e}, back-projection {rx:.2e}",
               flush=True)
         assert rv <= 1e-5 and rx <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["healpix", "healpix-minus-10pct", "random", "exact-pm"])
+@pytest.mark.parametrize("n_sat", [5, 14])
+def test_pair_layouts_snapshots_against_oracle(ops, kind, n_sat):
+    """The snapshot-structured kernels over the same layouts (5 satellites:
+    the 10-satellite register tile; 14: the 16-satellite one)."""
+    rng = np.random.default_rng(n_sat * 7 + len(kind))
+    lmn = sky_of(kind, rng)
+    P = lmn.shape[1]
+    I = rng.lognormal(0.0, 1.0, P)
+    pos = geo.satellite_positions(n_sat, 12, 30e3, True, 15, rng)
+    lam = geo.C_LIGHT / 10e6
+    uvw, xa, xb, _ = geo.baseline_rows(pos, lam)
+    K = uvw.shape[1]
+    y = rng.standard_normal(K) + 1j * rng.standard_normal(K)
+    wt = rng.uniform(0.5, 1.5, K)
+    c = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    sky = ops.Sky(c(lmn))
+    l, m, n = lmn
+    for mask, track, herm in ((True, False, False), (True, True, True), (False, False, False)):
+        flags = (oracle.MASK if mask else 0) | (oracle.PHASE_TRACK if track else 0) | (oracle.HERMITIAN if herm else 0)
+        kw = dict(mask=mask, phase_track=track, hermitian=herm, R_moon=R)
+        Vo = oracle.forward(l, m, n, I, uvw, xa, xb, flags=flags, R_moon=R)
+        V = sky.forward_snapshots(c(I), c(pos), lam, **kw).cpu().numpy()
+        xo = oracle.backproject(l, m, n, uvw, y, wt, xa, xb, flags=flags, R_moon=R)
+        x = sky.backproject_snapshots(c(pos), lam, c(y), c(wt), **kw).cpu().numpy()
+        rv, rx = rel_l2(V, Vo), rel_l2(x, xo)
+        print(f"{kind} n_sat={n_sat} mask={mask} track={track} herm={herm}: snapshot forward {rv:.2e}, "
+              f"back-projection {rx:.2e}", flush=True)
+        assert rv <= 1e-5 and rx <= 1e-5

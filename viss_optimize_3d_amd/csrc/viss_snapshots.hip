@@ -272,6 +272,61 @@ __device__ __forceinline__ void fwd_snapshot_tile(const PixPack& q, const SatRec
   }
 }
 
+// Sums a snapshot's FP32 row partials over the wavefront's lanes (xor
+// butterfly: every lane ends with every total); lane j keeps rows j and j + 64
+// and writes them (phase-track rotation, Hermitian rows), so the rows are
+// written by the wavefront in parallel.
+template <int NMAX>
+__device__ __forceinline__ void snapshot_store(float (&ar)[NMAX * (NMAX - 1) / 2], float (&ai)[NMAX * (NMAX - 1) / 2],
+                                               int lane, int64_t s, int64_t S, int N,
+                                               const SatRec* __restrict__ R, double inv_lambda, int track, int herm,
+                                               double* __restrict__ V) {
+  constexpr int NBM = NMAX * (NMAX - 1) / 2;
+  constexpr int NSLOT = (NBM + 63) / 64;
+  float myr[NSLOT], myi[NSLOT];
+#pragma unroll
+  for (int u = 0; u < NSLOT; ++u) myr[u] = myi[u] = 0.f;
+#pragma unroll
+  for (int q = 0; q < NBM; ++q) {
+    float vr = ar[q], vi = ai[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      vr += __shfl_xor(vr, o, 64);
+      vi += __shfl_xor(vi, o, 64);
+    }
+    if (lane == (q & 63)) {
+      myr[q >> 6] = vr;
+      myi[q >> 6] = vi;
+    }
+  }
+  const int NB = N * (N - 1) / 2;
+  const int64_t K = S * NB;
+#pragma unroll
+  for (int u = 0; u < NSLOT; ++u) {
+    const int q = u * 64 + lane;
+    if (q >= NBM) break;
+    int a, b;
+    pair_of(q, NMAX, a, b);
+    if (b >= N) continue;  // a padding satellite
+    const int64_t k = s * NB + row_in_snapshot(a, b, N);
+    double vr = myr[u], vi = myi[u];
+    if (track) {  // V e^{+2 pi i w}, w = (z_a - z_b) / lambda
+      const double w = (R[a].z - R[b].z) * inv_lambda;
+      double sn, cs;
+      sincospi(2.0 * (w - rint(w)), &sn, &cs);
+      const double nr = vr * cs - vi * sn, ni = vr * sn + vi * cs;
+      vr = nr;
+      vi = ni;
+    }
+    unsafeAtomicAdd(&V[2 * k], vr);
+    unsafeAtomicAdd(&V[2 * k + 1], vi);
+    if (herm) {
+      unsafeAtomicAdd(&V[2 * (k + K)], vr);
+      unsafeAtomicAdd(&V[2 * (k + K) + 1], -vi);
+    }
+  }
+}
+
 // Forward over snapshot-structured rows: V_ab(s) = sum_p I_p conj(G_a(p)) G_b(p)
 // (= sum_p I_p m_kp exp(-2 pi i b . s_p)).  One wavefront per (snapshot,
 // pixel range); each lane owns one pixel of every 64-pixel tile of the range
@@ -322,52 +377,284 @@ __global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(
       fwd_snapshot_tile<true, NMAX>(pix[t * kTile + lane], R, inv_lambda, ar, ai);
     }
   }
-  // Sum over the wavefront's pixels (xor butterfly: every lane ends with every
-  // total); lane j keeps pairs j and j + 64 and writes them, so the rows are
-  // written by the wavefront in parallel.
-  constexpr int NSLOT = (NBM + 63) / 64;
-  float myr[NSLOT], myi[NSLOT];
+  snapshot_store<NMAX>(ar, ai, lane, s, S, N, R, inv_lambda, track, herm, V);
+}
+
+// ------------------------------------------------ antipodal-pair sky (mixed)
+// For a pair (a at s, b at -s), r_i.s_b = -(r_i.s_a): the b pixel's phasor is
+// the conjugate of a's, g'_i = conj(g_i), masked by its own test -D_i >= t_i.
+// Where every satellite sees both pixels of every pair of a tile whole, the
+// two pixels share the phasors and the FP32 row work:
+//   forward  V_ij += I_a conj(g_i) g_j + I_b g_i conj(g_j)
+//                 = (I_a + I_b) Re F_ij + i (I_a - I_b) Im F_ij,  F_ij = conj(g_i) g_j;
+//   adjoint  with P_i = sum_j y_ij Re g_j, Q_i = sum_j y_ij Im g_j (complex),
+//            H_i = P_i - i Q_i for a and H'_i = P_i + i Q_i for b.
+// Modes of a (snapshot, pair tile): 0 both sides seen whole, 1 only a (b
+// hidden), 2 only b, 3 anything else (both sides with the per-pixel test).
+enum SnapPairMode { kSnapBoth = 0, kSnapA = 1, kSnapB = 2, kSnapMasked = 3 };
+
+template <int NMAX>
+__device__ __forceinline__ void classify_snapshot_pair(const float4 cn, const SatCls* __restrict__ c, int N,
+                                                       bool& anyA, bool& allA, bool& anyB, bool& allB) {
+  anyA = anyB = false;
+  allA = allB = true;
 #pragma unroll
-  for (int u = 0; u < NSLOT; ++u) myr[u] = myi[u] = 0.f;
-#pragma unroll
-  for (int q = 0; q < NBM; ++q) {
-    float vr = ar[q], vi = ai[q];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      vr += __shfl_xor(vr, o, 64);
-      vi += __shfl_xor(vi, o, 64);
-    }
-    if (lane == (q & 63)) {
-      myr[q >> 6] = vr;
-      myi[q >> 6] = vi;
-    }
-  }
-  const int NB = N * (N - 1) / 2;
-  const int64_t K = S * NB;
-#pragma unroll
-  for (int u = 0; u < NSLOT; ++u) {
-    const int q = u * 64 + lane;
-    if (q >= NBM) break;
-    int a, b;
-    pair_of(q, NMAX, a, b);
-    if (b >= N) continue;  // a padding satellite
-    const int64_t k = s * NB + row_in_snapshot(a, b, N);
-    double vr = myr[u], vi = myi[u];
-    if (track) {  // V e^{+2 pi i w}, w = (z_a - z_b) / lambda
-      const double w = (R[a].z - R[b].z) * inv_lambda;
-      double sn, cs;
-      sincospi(2.0 * (w - rint(w)), &sn, &cs);
-      const double nr = vr * cs - vi * sn, ni = vr * sn + vi * cs;
-      vr = nr;
-      vi = ni;
-    }
-    unsafeAtomicAdd(&V[2 * k], vr);
-    unsafeAtomicAdd(&V[2 * k + 1], vi);
-    if (herm) {
-      unsafeAtomicAdd(&V[2 * (k + K)], vr);
-      unsafeAtomicAdd(&V[2 * (k + K) + 1], -vi);
+  for (int i = 0; i < NMAX; ++i) {
+    if (i < N) {
+      const SatCls q = c[i];
+      const float d = fmaf(cn.x, q.ax, fmaf(cn.y, q.ay, cn.z * q.az));
+      anyA |= d + cn.w >= q.c - kClsEps;
+      allA &= d - cn.w >= q.c + kClsEps;
+      anyB |= -d + cn.w >= q.c - kClsEps;
+      allB &= -d - cn.w >= q.c + kClsEps;
     }
   }
+}
+
+// Phasors of pixel a: (gr, gi) unmasked; in kSnapMasked also the masked a-side
+// phasors (ar_, ai_) and b-side phasors (br_, bi_) = [-D >= t] conj(g).
+template <int MODE, int NMAX>
+__device__ __forceinline__ void pair_phasors(const SatRec* __restrict__ R, double l, double m, double n,
+                                             double inv_lambda, float (&gr)[NMAX], float (&gi)[NMAX],
+                                             float (&ga_r)[NMAX], float (&ga_i)[NMAX], float (&gb_r)[NMAX],
+                                             float (&gb_i)[NMAX]) {
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    const SatRec q = R[i];
+    const double D = fma(q.x, l, fma(q.y, m, q.z * n));
+    float sn, cs;
+    sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
+    gr[i] = cs;
+    gi[i] = sn;
+    if (MODE == kSnapMasked) {
+      const bool va = D >= q.t, vb = -D >= q.t;
+      ga_r[i] = va ? cs : 0.f;
+      ga_i[i] = va ? sn : 0.f;
+      gb_r[i] = vb ? cs : 0.f;
+      gb_i[i] = vb ? -sn : 0.f;
+    }
+  }
+}
+
+// One snapshot's contribution to the pair (a, b) of this lane.
+template <int MODE, int NMAX>
+__device__ __forceinline__ void bp_snapshot_pair(const SatRec* __restrict__ R, const float2* __restrict__ Y, double l,
+                                                 double m, double n, double inv_lambda, double& acc_a, double& acc_b) {
+  float gr[NMAX], gi[NMAX], ga_r[NMAX], ga_i[NMAX], gb_r[NMAX], gb_i[NMAX];
+  pair_phasors<MODE, NMAX>(R, l, m, n, inv_lambda, gr, gi, ga_r, ga_i, gb_r, gb_i);
+  float pa = 0.f, pb = 0.f;
+  int pr = 0;
+#pragma unroll
+  for (int i = 0; i < NMAX - 1; ++i) {
+    if (MODE == kSnapBoth) {
+      float Pr = 0.f, Pi = 0.f, Qr = 0.f, Qi = 0.f;
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j) {
+        const float2 y = Y[pr++];
+        Pr = fmaf(y.x, gr[j], Pr);
+        Pi = fmaf(y.y, gr[j], Pi);
+        Qr = fmaf(y.x, gi[j], Qr);
+        Qi = fmaf(y.y, gi[j], Qi);
+      }
+      pa = fmaf(gr[i], Pr + Qi, fmaf(-gi[i], Pi - Qr, pa));  // Re g_i (P - iQ)
+      pb = fmaf(gr[i], Pr - Qi, fmaf(gi[i], Pi + Qr, pb));   // Re conj(g_i) (P + iQ)
+    } else if (MODE == kSnapA) {
+      float hr = 0.f, hi = 0.f;
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j) {  // H += y conj(g_j)
+        const float2 y = Y[pr++];
+        hr = fmaf(y.x, gr[j], fmaf(y.y, gi[j], hr));
+        hi = fmaf(y.y, gr[j], fmaf(-y.x, gi[j], hi));
+      }
+      pa = fmaf(gr[i], hr, fmaf(-gi[i], hi, pa));
+    } else if (MODE == kSnapB) {
+      float hr = 0.f, hi = 0.f;
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j) {  // H' += y g_j
+        const float2 y = Y[pr++];
+        hr = fmaf(y.x, gr[j], fmaf(-y.y, gi[j], hr));
+        hi = fmaf(y.y, gr[j], fmaf(y.x, gi[j], hi));
+      }
+      pb = fmaf(gr[i], hr, fmaf(gi[i], hi, pb));  // Re conj(g_i) H'
+    } else {
+      float har = 0.f, hai = 0.f, hbr = 0.f, hbi = 0.f;
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j) {  // H += y conj(G_j) for each side
+        const float2 y = Y[pr++];
+        har = fmaf(y.x, ga_r[j], fmaf(y.y, ga_i[j], har));
+        hai = fmaf(y.y, ga_r[j], fmaf(-y.x, ga_i[j], hai));
+        hbr = fmaf(y.x, gb_r[j], fmaf(y.y, gb_i[j], hbr));
+        hbi = fmaf(y.y, gb_r[j], fmaf(-y.x, gb_i[j], hbi));
+      }
+      pa = fmaf(ga_r[i], har, fmaf(-ga_i[i], hai, pa));
+      pb = fmaf(gb_r[i], hbr, fmaf(-gb_i[i], hbi, pb));
+    }
+  }
+  acc_a += (double)pa;
+  acc_b += (double)pb;
+}
+
+// The adjoint over a paired sky: one wavefront per 64-pair tile, one pair per
+// lane; snapshots classified 64 at a time (one per lane) against both sides of
+// the tile and run in the four mode loops.
+template <bool MASK, int NMAX>
+__global__ __launch_bounds__(256) void k_backproject_snap_pairs(const PairPack* __restrict__ pp,
+                                                                const int2* __restrict__ pidx,
+                                                                const float4* __restrict__ pcone, int64_t ntile,
+                                                                const SatRec* __restrict__ rec,
+                                                                const SatCls* __restrict__ cls,
+                                                                const float2* __restrict__ yw, int64_t S, int N,
+                                                                double inv_lambda, int64_t s_per_chunk, double herm,
+                                                                double* __restrict__ x) {
+  constexpr int NBP = NMAX * (NMAX - 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
+  if (tile >= ntile) return;  // wave-uniform
+  const PairPack q = pp[tile * 64 + lane];
+  const int2 ab = pidx[tile * 64 + lane];
+  float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
+  if (MASK) cn = pcone[tile];
+  const int64_t s0 = (int64_t)blockIdx.y * s_per_chunk;
+  const int64_t s1 = min(S, s0 + s_per_chunk);
+  double acc_a = 0.0, acc_b = 0.0;
+  for (int64_t sb = s0; sb < s1; sb += 64) {
+    const int ns = (int)min((int64_t)64, s1 - sb);
+    uint64_t both, onlyA, onlyB, masked;
+    if (MASK) {
+      bool anyA = false, allA = false, anyB = false, allB = false;
+      if (lane < ns) classify_snapshot_pair<NMAX>(cn, cls + (sb + lane) * NMAX, N, anyA, allA, anyB, allB);
+      const uint64_t sA = __ballot(anyA), fA = __ballot(allA), sB = __ballot(anyB), fB = __ballot(allB);
+      both = fA & fB;
+      onlyA = fA & ~sB;
+      onlyB = fB & ~sA;
+      masked = (sA | sB) & ~(both | onlyA | onlyB);
+    } else {
+      both = ns == 64 ? ~0ull : ((1ull << ns) - 1ull);
+      onlyA = onlyB = masked = 0ull;
+    }
+    while (both) {
+      const int64_t s = sb + __builtin_ctzll(both);
+      both &= both - 1ull;
+      bp_snapshot_pair<kSnapBoth, NMAX>(rec + s * NMAX, yw + s * NBP, q.l, q.m, q.n, inv_lambda, acc_a, acc_b);
+    }
+    while (onlyA) {
+      const int64_t s = sb + __builtin_ctzll(onlyA);
+      onlyA &= onlyA - 1ull;
+      bp_snapshot_pair<kSnapA, NMAX>(rec + s * NMAX, yw + s * NBP, q.l, q.m, q.n, inv_lambda, acc_a, acc_b);
+    }
+    while (onlyB) {
+      const int64_t s = sb + __builtin_ctzll(onlyB);
+      onlyB &= onlyB - 1ull;
+      bp_snapshot_pair<kSnapB, NMAX>(rec + s * NMAX, yw + s * NBP, q.l, q.m, q.n, inv_lambda, acc_a, acc_b);
+    }
+    while (masked) {
+      const int64_t s = sb + __builtin_ctzll(masked);
+      masked &= masked - 1ull;
+      bp_snapshot_pair<kSnapMasked, NMAX>(rec + s * NMAX, yw + s * NBP, q.l, q.m, q.n, inv_lambda, acc_a, acc_b);
+    }
+  }
+  if (ab.x >= 0) unsafeAtomicAdd(&x[ab.x], herm * acc_a);
+  if (ab.y >= 0) unsafeAtomicAdd(&x[ab.y], herm * acc_b);
+}
+
+// One pair tile's contribution to a snapshot's row sums (one pair per lane).
+template <int MODE, int NMAX>
+__device__ __forceinline__ void fwd_snapshot_pair_tile(const PairPack& q, const SatRec* __restrict__ R,
+                                                       double inv_lambda, float (&ar)[NMAX * (NMAX - 1) / 2],
+                                                       float (&ai)[NMAX * (NMAX - 1) / 2]) {
+  float gr[NMAX], gi[NMAX], ga_r[NMAX], ga_i[NMAX], gb_r[NMAX], gb_i[NMAX];
+  pair_phasors<MODE, NMAX>(R, q.l, q.m, q.n, inv_lambda, gr, gi, ga_r, ga_i, gb_r, gb_i);
+  int pr = 0;
+#pragma unroll
+  for (int i = 0; i < NMAX - 1; ++i) {
+    if (MODE == kSnapBoth) {  // (I_a + I_b) Re F + i (I_a - I_b) Im F
+      const float wAr = q.A * gr[i], wAi = q.A * gi[i], wDr = q.D * gr[i], wDi = q.D * gi[i];
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j, ++pr) {
+        ar[pr] = fmaf(wAr, gr[j], fmaf(wAi, gi[j], ar[pr]));
+        ai[pr] = fmaf(wDr, gi[j], fmaf(-wDi, gr[j], ai[pr]));
+      }
+    } else if (MODE == kSnapA) {  // I_a conj(g_i) g_j
+      const float fr = q.Ia * gr[i], fi = -q.Ia * gi[i];
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j, ++pr) {
+        ar[pr] = fmaf(fr, gr[j], fmaf(-fi, gi[j], ar[pr]));
+        ai[pr] = fmaf(fr, gi[j], fmaf(fi, gr[j], ai[pr]));
+      }
+    } else if (MODE == kSnapB) {  // I_b g_i conj(g_j)
+      const float fr = q.Ib * gr[i], fi = q.Ib * gi[i];
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j, ++pr) {
+        ar[pr] = fmaf(fr, gr[j], fmaf(fi, gi[j], ar[pr]));
+        ai[pr] = fmaf(-fr, gi[j], fmaf(fi, gr[j], ai[pr]));
+      }
+    } else {  // I_a conj(G_i) G_j + I_b conj(G'_i) G'_j with the masked phasors of each side
+      const float far_ = q.Ia * ga_r[i], fai = -q.Ia * ga_i[i];
+      const float fbr = q.Ib * gb_r[i], fbi = -q.Ib * gb_i[i];
+#pragma unroll
+      for (int j = i + 1; j < NMAX; ++j, ++pr) {
+        ar[pr] = fmaf(far_, ga_r[j], fmaf(-fai, ga_i[j], fmaf(fbr, gb_r[j], fmaf(-fbi, gb_i[j], ar[pr]))));
+        ai[pr] = fmaf(far_, ga_i[j], fmaf(fai, ga_r[j], fmaf(fbr, gb_i[j], fmaf(fbi, gb_r[j], ai[pr]))));
+      }
+    }
+  }
+}
+
+// The forward over a paired sky: one wavefront per (snapshot, range of pair
+// tiles), one pair per lane; tiles classified 64 at a time (one per lane)
+// against both sides.  Epilogue as k_forward_snap.
+template <bool MASK, int NMAX>
+__global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap_pairs(
+    const PairPack* __restrict__ pp, const float4* __restrict__ pcone, int64_t ntile, int64_t tiles_per_range,
+    const SatRec* __restrict__ rec, const SatCls* __restrict__ cls, int64_t S, int N, double inv_lambda, int track,
+    int herm, double* __restrict__ V) {
+  constexpr int NBM = NMAX * (NMAX - 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + wave_in_block();
+  if (s >= S) return;  // wave-uniform
+  const int64_t t0 = (int64_t)blockIdx.y * tiles_per_range;
+  const int64_t t1 = min(ntile, t0 + tiles_per_range);
+  const SatRec* __restrict__ R = rec + s * NMAX;
+  float ar[NBM], ai[NBM];
+#pragma unroll
+  for (int k = 0; k < NBM; ++k) ar[k] = ai[k] = 0.f;
+  for (int64_t tb = t0; tb < t1; tb += 64) {
+    const int nt = (int)min((int64_t)64, t1 - tb);
+    uint64_t both, onlyA, onlyB, masked;
+    if (MASK) {
+      bool anyA = false, allA = false, anyB = false, allB = false;
+      if (lane < nt) classify_snapshot_pair<NMAX>(pcone[tb + lane], cls + s * NMAX, N, anyA, allA, anyB, allB);
+      const uint64_t sA = __ballot(anyA), fA = __ballot(allA), sB = __ballot(anyB), fB = __ballot(allB);
+      both = fA & fB;
+      onlyA = fA & ~sB;
+      onlyB = fB & ~sA;
+      masked = (sA | sB) & ~(both | onlyA | onlyB);
+    } else {
+      both = nt == 64 ? ~0ull : ((1ull << nt) - 1ull);
+      onlyA = onlyB = masked = 0ull;
+    }
+    while (both) {
+      const int64_t t = tb + __builtin_ctzll(both);
+      both &= both - 1ull;
+      fwd_snapshot_pair_tile<kSnapBoth, NMAX>(pp[t * 64 + lane], R, inv_lambda, ar, ai);
+    }
+    while (onlyA) {
+      const int64_t t = tb + __builtin_ctzll(onlyA);
+      onlyA &= onlyA - 1ull;
+      fwd_snapshot_pair_tile<kSnapA, NMAX>(pp[t * 64 + lane], R, inv_lambda, ar, ai);
+    }
+    while (onlyB) {
+      const int64_t t = tb + __builtin_ctzll(onlyB);
+      onlyB &= onlyB - 1ull;
+      fwd_snapshot_pair_tile<kSnapB, NMAX>(pp[t * 64 + lane], R, inv_lambda, ar, ai);
+    }
+    while (masked) {
+      const int64_t t = tb + __builtin_ctzll(masked);
+      masked &= masked - 1ull;
+      fwd_snapshot_pair_tile<kSnapMasked, NMAX>(pp[t * 64 + lane], R, inv_lambda, ar, ai);
+    }
+  }
+  snapshot_store<NMAX>(ar, ai, lane, s, S, N, R, inv_lambda, track, herm, V);
 }
 
 static int64_t cdiv_s(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -421,7 +708,8 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
   k_snap_rows<<<grid_rows(J), 256, 0, s>>>(sat_pos, y, wt, S, N, NP, inv_lambda, (flags & VISS_PHASE_TRACK) ? 1 : 0,
                                            yw);
   VISS_LAUNCHED(k_snap_rows);
-  const int64_t bx = cdiv_s(sky->nbtile, 4);
+  const bool pairs = sky->use_pairs;  // one lane per antipodal pair
+  const int64_t bx = cdiv_s(pairs ? sky->nptile : sky->nbtile, 4);
   // about 4096 rows per chunk, as the per-row kernel (VISS_BP_ROWS overrides it
   // for launch-shape sweeps); more chunks for small launches
   int64_t rows_per_chunk = 4096;
@@ -434,9 +722,15 @@ extern "C" int viss_sky_backproject_snapshots(viss_sky* sky, const double* sat_p
   if (cdiv_s(S, spc) > 65535) spc = cdiv_s(S, 65535);
   dim3 grid((unsigned)bx, (unsigned)cdiv_s(S, spc));
   const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
-#define VISS_BPS(M, NM)                                                                                             \
-  k_backproject_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, rec, cls, yw, S, N, inv_lambda, \
-                                                 spc, herm, x)
+#define VISS_BPS(M, NM)                                                                                              \
+  do {                                                                                                               \
+    if (pairs)                                                                                                       \
+      k_backproject_snap_pairs<M, NM><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, rec, cls, \
+                                                           yw, S, N, inv_lambda, spc, herm, x);                     \
+    else                                                                                                             \
+      k_backproject_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, rec, cls, yw, S, N,          \
+                                                     inv_lambda, spc, herm, x);                                      \
+  } while (0)
   if (NP == 4) {
     if (mask) VISS_BPS(true, 4); else VISS_BPS(false, 4);
   } else if (NP == 8) {
@@ -489,19 +783,27 @@ extern "C" int viss_sky_forward_snapshots(viss_sky* sky, const double* I, const 
   // 256 tiles (16,384 pixels, 512 KB of pixel data) per range, fewer when the
   // snapshots alone would not fill the device (VISS_SNAP_TILES overrides the
   // starting length for launch-shape sweeps)
+  const bool pairs = sky->use_pairs;  // one lane per antipodal pair, tiles of 64 pairs
+  const int64_t ntile = pairs ? sky->nptile : sky->ntile;
   int64_t tpr = 256;
   if (const char* e = getenv("VISS_SNAP_TILES")) {
     const long long v = atoll(e);
     if (v > 0) tpr = v;
   }
-  while (tpr > 16 && bx * cdiv_s(sky->ntile, tpr) < 4096) tpr /= 2;
-  tpr = std::max(tpr, cdiv_s(sky->ntile, 65535));
+  while (tpr > 16 && bx * cdiv_s(ntile, tpr) < 4096) tpr /= 2;
+  tpr = std::max(tpr, cdiv_s(ntile, 65535));
   VISS_REQUIRE(bx <= INT32_MAX, "viss_sky_forward_snapshots: too many snapshots");
-  dim3 grid((unsigned)bx, (unsigned)cdiv_s(sky->ntile, tpr));
+  dim3 grid((unsigned)bx, (unsigned)cdiv_s(ntile, tpr));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
-#define VISS_FS(M, NM)                                                                                             \
-  k_forward_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->cone, sky->ntile, tpr, rec, cls, S, N, inv_lambda, track, \
-                                             herm, V)
+#define VISS_FS(M, NM)                                                                                               \
+  do {                                                                                                               \
+    if (pairs)                                                                                                       \
+      k_forward_snap_pairs<M, NM><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, sky->nptile, tpr, rec, cls, S, N,     \
+                                                       inv_lambda, track, herm, V);                                  \
+    else                                                                                                             \
+      k_forward_snap<M, NM><<<grid, 256, 0, s>>>(sky->pix, sky->cone, sky->ntile, tpr, rec, cls, S, N, inv_lambda,   \
+                                                 track, herm, V);                                                    \
+  } while (0)
   if (NP == 4) {
     if (mask) VISS_FS(true, 4); else VISS_FS(false, 4);
   } else if (NP == 8) {
