@@ -90,8 +90,8 @@ def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
     out = [f"calibration: {calib_mib} MiB copy reads FETCH_SIZE={max(copies_f):.1f} KiB, WRITE_SIZE={max(copies_w):.1f} KiB",
            f"scale factors: fetch x{ff:.4f}, write x{fw:.4f}"]
     keys = ("k_forward_pairs<true>", "k_backproject_pairs<true>", "k_forward<true, false>",
-            "k_backproject<true, false>", "k_forward_snap<true, 10>",
-            "k_backproject_snap<true, 10>", "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
+            "k_backproject<true, false>", "k_forward_snap_pairs<true, 10>", "k_backproject_snap_pairs<true, 10>",
+            "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
             "k_pix2ang", "k_pixel_dirs",
             "k_bp_prologue", "k_shell_hist", "k_row_weights")
     for key in keys:
