@@ -254,16 +254,20 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
       both = nk == 64 ? ~0ull : ((1ull << nk) - 1ull);
       onlyA = onlyB = mixed = 0ull;
     }
-    float sa = 0.f, sb = 0.f;
+    // Rows that see both pixels whole: t_a = P - Q, t_b = P + Q with
+    // P = sum yr cos, Q = sum yi sin (two FMAs per row for both pixels).
+    float sa = 0.f, sb = 0.f, P = 0.f, Q = 0.f;
     while (both) {
       const int j = __builtin_ctzll(both);
       both &= both - 1ull;
       const VisHot h = hot[kb + j];
       float sn, cs;
       sincos_rev(rev_fp32(fma(h.u, l, fma(h.v, m, h.w * n))), sn, cs);
-      sa = fmaf(h.yr, cs, fmaf(-h.yi, sn, sa));
-      sb = fmaf(h.yr, cs, fmaf(h.yi, sn, sb));
+      P = fmaf(h.yr, cs, P);
+      Q = fmaf(h.yi, sn, Q);
     }
+    sa += P - Q;
+    sb += P + Q;
     while (onlyA) {
       const int j = __builtin_ctzll(onlyA);
       onlyA &= onlyA - 1ull;
