@@ -59,8 +59,14 @@ struct VisHot {  // mixed precision: phase in FP64, weighted visibility in FP32
 };
 static_assert(sizeof(VisHot) == 32, "VisHot is 32 B");
 
-struct VisHot64 {  // all-FP64 path
-  double u, v, w, yr, yi, pad0, pad1, pad2;
+// All-FP64 path.  x is real, so a row adds Re(y e^{2 pi i t}) =
+// ya cos(2 pi (t + ph)) with y = ya e^{2 pi i ph}: one FP64 cospi per
+// (row, pixel) instead of sincospi.
+struct VisHot64 {
+  double u, v, w, yr, yi;
+  double ph;  // arg(y) / (2 pi), revolutions in (-1/2, 1/2]
+  double ya;  // |y|
+  double pad;
 };
 static_assert(sizeof(VisHot64) == 64, "VisHot64 is 64 B");
 
