@@ -127,7 +127,11 @@ int viss_weights(const float* bll, const double* w, int64_t K, int32_t nr, int f
  * as HEALPix pixels are: the mixed-precision per-row forward and
  * back-projection then evaluate one phase and one sin/cos per pair.  Skies
  * with fewer than half of their pixels paired use the per-pixel kernels.
- * Creation synchronises the stream once (to size the pair arrays). */
+ * Pairs whose first pixels are (x, y, z) and (-x, -y, z) form quads
+ * {(+-x, +-y, +-z)} (HEALPix is symmetric under z -> -z ring by ring): the
+ * mixed-precision per-row forward then evaluates one sin/cos per quad and
+ * applies each ring's factor exp(-2 pi i w z) once per tile of 16 quads.
+ * Creation synchronises the stream a few times (to size the layouts). */
 typedef struct viss_sky viss_sky;
 
 int viss_sky_create(const double* lmn, int64_t P, viss_sky** sky, void* stream);
@@ -136,6 +140,10 @@ int64_t viss_sky_npix(const viss_sky* sky);
 /* Antipodal pairs and unpaired pixels found at creation (host pointers,
  * nullable); *uses_pairs = 1 when the mixed per-row kernels use the pairs. */
 int viss_sky_pairs(const viss_sky* sky, int64_t* npair, int64_t* nsingle, int* uses_pairs);
+/* Equatorial-mirror quads found at creation, their 16-quad tiles and
+ * whether the mixed per-row forward uses them (host pointers, nullable).
+ * Layout query only: no counterpart in the reference. */
+int viss_sky_quads(const viss_sky* sky, int64_t* nquad, int64_t* nquad_tiles, int* uses_quads);
 
 int viss_sky_forward(viss_sky* sky, const double* I, const double* uvw, const double* xyz_a, const double* xyz_b,
                      int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream);

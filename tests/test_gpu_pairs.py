@@ -128,3 +128,63 @@ def test_pair_layouts_snapshots_against_oracle(ops, kind, n_sat):
         print(f"{kind} n_sat={n_sat} mask={mask} track={track} herm={herm}: snapshot forward {rv:.2e}, "
               f"back-projection {rx:.2e}", flush=True)
         assert rv <= 1e-5 and rx <= 1e-5
+
+
+def ring_sky(rng, nring=20, per_ring=48):
+    """Quads on rings: for each of nring heights z, per_ring azimuths in [0, pi)
+    and the four mirror images (x, y, +-z), (-x, -y, +-z)."""
+    out = []
+    for z in rng.uniform(0.05, 0.95, nring):
+        phi = rng.uniform(0.0, np.pi, per_ring)
+        r = np.sqrt(1.0 - z * z)
+        x, y = r * np.cos(phi), r * np.sin(phi)
+        zz = np.full(per_ring, z)
+        for sx, sz in ((1, 1), (-1, 1), (1, -1), (-1, -1)):
+            out.append(np.stack([sx * x, sx * y, sz * zz]))
+    lmn = np.concatenate(out, axis=1)
+    return lmn[:, rng.permutation(lmn.shape[1])]
+
+
+def test_quad_counts(ops):
+    """HEALPix: every pair off the equator ring is in a quad; the tiles are
+    full enough for the quad forward from nside 8 on."""
+    rng = np.random.default_rng(4)
+    for nside, nest in ((16, False), (8, True), (64, False)):
+        th, ph, _, _ = oracle.pix2ang(nside, nest)
+        lmn = np.stack(oracle.pixel_dirs(th, ph, np.ones_like(th), 1.0)[:3])
+        got = ops.Sky(torch.from_numpy(lmn).cuda()).pairs()
+        print(f"healpix nside={nside} nest={nest}: {got}", flush=True)
+        P = lmn.shape[1]
+        assert got["pairs"] == P // 2 and got["quads"] == (P // 2 - 2 * nside) // 2 and got["uses_quads"]
+    got = ops.Sky(torch.from_numpy(ring_sky(rng)).cuda()).pairs()
+    print(f"rings: {got}", flush=True)
+    assert got["quads"] == 20 * 48 and got["quad_tiles"] == 20 * 3 and got["uses_quads"]
+    got = ops.Sky(torch.from_numpy(sky_of("exact-pm", rng)).cuda()).pairs()
+    assert got["quads"] == 0 and not got["uses_quads"]
+
+
+@pytest.mark.parametrize("kind", ["rings", "rings-minus"])
+def test_quad_layouts_against_oracle(ops, kind):
+    """Forward over quad skies that are not HEALPix (rings of random azimuths;
+    with 5% of the pixels removed: quads, leftover pairs and singles)."""
+    rng = np.random.default_rng(11 if kind == "rings" else 12)
+    lmn = ring_sky(rng)
+    if kind == "rings-minus":
+        keep = np.sort(rng.choice(lmn.shape[1], int(lmn.shape[1] * 0.95), replace=False))
+        lmn = lmn[:, keep]
+    P = lmn.shape[1]
+    I = rng.lognormal(0.0, 1.0, P)
+    c = make_case(nside=4, n_snap=40, n_sat=6, freq=10e6, sigma=20e3, seed=5)
+    uvw, xa, xb = c["uvw"], c["xa"], c["xb"]
+    sky = ops.Sky(torch.from_numpy(lmn).cuda())
+    info = sky.pairs()
+    print(f"{kind}: {info}", flush=True)
+    t = {k: torch.from_numpy(v).cuda() for k, v in dict(uvw=uvw, xa=xa, xb=xb, I=I).items()}
+    l, m, n = lmn
+    for mask, track, herm in ((True, False, False), (True, True, True), (False, False, False)):
+        flags = (oracle.MASK if mask else 0) | (oracle.PHASE_TRACK if track else 0) | (oracle.HERMITIAN if herm else 0)
+        Vo = oracle.forward(l, m, n, I, uvw, xa, xb, flags=flags, R_moon=R)
+        V = sky.forward(t["I"], t["uvw"], t["xa"], t["xb"], mask=mask, phase_track=track, hermitian=herm, R_moon=R)
+        rv = rel_l2(V.cpu().numpy(), Vo)
+        print(f"{kind} mask={mask} track={track} herm={herm}: forward {rv:.2e}", flush=True)
+        assert rv <= 1e-5

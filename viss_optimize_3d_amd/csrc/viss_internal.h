@@ -53,6 +53,18 @@ struct PairPack {
 };
 static_assert(sizeof(PairPack) == 40, "PairPack is 40 B");
 
+// Quad of the equatorial-mirror layout: pixels a = (x, y, z), b = (-x, -y, z)
+// on ring z and c = (x, y, -z), d = (-x, -y, -z) on ring -z (z is the tile's).
+// With A = u x + v y and C = w z their phases are A + C, -A + C, A - C, -A - C,
+// so one sin/cos of A serves all four; Az/Dz = Ia +- Ib, Am/Dm = Ic +- Id.
+struct QuadPack {
+  double x, y;
+  float Az, Dz, Am, Dm;
+  float Ia, Ib, Ic, Id;
+};
+static_assert(sizeof(QuadPack) == 48, "QuadPack is 48 B");
+constexpr int kQuadTile = 16;  // quads per tile, all on one ring pair
+
 struct VisHot {  // mixed precision: phase in FP64, weighted visibility in FP32
   double u, v, w;
   float yr, yi;
@@ -143,6 +155,21 @@ struct viss_sky {
   viss::PairPack* ppack = nullptr;  // [nptile * 64]
   int2* pidx = nullptr;             // [nptile * 64] caller indices of (a, b); -1 if absent
   float4* pcone = nullptr;          // [nptile] bounding cones of the a pixels of each tile
+  // Equatorial-mirror quads (mixed-precision per-row forward): two antipodal
+  // pairs whose a pixels are (x, y, z) and (-x, -y, z).  Tiles of kQuadTile
+  // quads sharing one z, sorted by z then azimuth; the pairs and singles left
+  // over form a second pair layout (rest).
+  int64_t nquad = 0;                // quads found
+  int64_t nqtile = 0;               // kQuadTile-quad tiles
+  bool use_quads = false;           // quads cover half the pixels, tiles at least 2/3 full
+  viss::QuadPack* qpack = nullptr;  // [nqtile * kQuadTile]
+  int4* qidx = nullptr;             // [nqtile * kQuadTile] caller indices of (a, b, c, d); -1 if absent
+  double* qz = nullptr;             // [nqtile] z of each tile
+  float4* qcone = nullptr;          // [2 nqtile] cones of the a and b pixels of each tile
+  int64_t nrtile = 0;               // 64-entry tiles of the rest layout
+  viss::PairPack* rpack = nullptr;  // [nrtile * 64]
+  int2* ridx = nullptr;             // [nrtile * 64]
+  float4* rcone = nullptr;          // [nrtile]
   viss::DevBuf ws;               // per-call visibility packs
 };
 

@@ -262,6 +262,123 @@ __device__ __forceinline__ void forward_pair_tiles(const PairPack* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------ forward, equatorial-mirror quads
+// Quad tile at height z (viss_sky's quad layout): A = u x + v y per quad and
+// C = w z per tile.  Ring z holds a = (x, y, z), b = (-x, -y, z) with phases
+// A + C, -A + C; ring -z holds c = (x, y, -z), d = (-x, -y, -z) with A - C,
+// -A - C.  Per ring, R = sum (wa + wb) cos A and J = sum (wa - wb) sin A, and
+// the tile adds e^{-i C}(Rz - i Jz) + e^{+i C}(Rm - i Jm).
+// Mode of one ring for every lane of the wavefront: nothing seen, both
+// pixels seen whole, only the first (a or c), only the second (b or d), or
+// the exact per-pixel test.
+enum RingMode { kRingNone = 0, kRingFF = 1, kRingFN = 2, kRingNF = 3, kRingTest = 4 };
+
+// Per-lane thresholds of a tile: pixel (x, y, z) passes satellite a when
+// x xa + y ya >= ta - z za = t1a, (-x, -y, z) when x xa + y ya <= -t1a,
+// (x, y, -z) when >= ta + z za = t2a and (-x, -y, -z) when <= -t2a.
+struct QuadThr {
+  double t1a, t1b, t2a, t2b;
+};
+
+template <int MR>
+__device__ __forceinline__ void ring_step(float w1, float w2, float sum, float dif, bool v1, bool v2, float cs,
+                                          float sn, float& R, float& J) {
+  if (MR == kRingFF) {
+    R = fmaf(sum, cs, R);
+    J = fmaf(dif, sn, J);
+  } else if (MR == kRingFN) {
+    R = fmaf(w1, cs, R);
+    J = fmaf(w1, sn, J);
+  } else if (MR == kRingNF) {
+    R = fmaf(w2, cs, R);
+    J = fmaf(-w2, sn, J);
+  } else if (MR == kRingTest) {
+    const float a = v1 ? w1 : 0.f, b = v2 ? w2 : 0.f;
+    R = fmaf(a + b, cs, R);
+    J = fmaf(a - b, sn, J);
+  }
+}
+
+template <int MZ, int MM>
+__device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ tq, double u, double v,
+                                                  const VisMask& mk, const QuadThr& th, float& Rz, float& Jz,
+                                                  float& Rm, float& Jm) {
+#pragma unroll 4
+  for (int j = 0; j < kQuadTile; ++j) {
+    const QuadPack q = tq[j];
+    float sn, cs;
+    sincos_rev(rev_fp32(fma(u, q.x, v * q.y)), sn, cs);
+    double pa = 0.0, pb = 0.0;
+    if (MZ == kRingTest || MM == kRingTest) {
+      pa = fma(q.x, mk.xa, q.y * mk.ya);
+      pb = fma(q.x, mk.xb, q.y * mk.yb);
+    }
+    ring_step<MZ>(q.Ia, q.Ib, q.Az, q.Dz, (pa >= th.t1a) & (pb >= th.t1b), (pa <= -th.t1a) & (pb <= -th.t1b), cs,
+                  sn, Rz, Jz);
+    ring_step<MM>(q.Ic, q.Id, q.Am, q.Dm, (pa >= th.t2a) & (pb >= th.t2b), (pa <= -th.t2a) & (pb <= -th.t2b), cs,
+                  sn, Rm, Jm);
+  }
+}
+
+__device__ __forceinline__ int ring_mode(bool see1, bool whole1, bool see2, bool whole2) {
+  const bool t1 = see1 & !whole1, t2 = see2 & !whole2;
+  return (!see1 & !see2) ? kRingNone
+         : (t1 | t2)     ? kRingTest
+         : (see1 & see2) ? kRingFF
+         : see1          ? kRingFN
+                         : kRingNF;
+}
+
+// Accumulates the quad tiles [t0, t1) into (Sr, Si) (the convention of
+// forward_store, without the phase-tracking term).
+template <bool MASK>
+__device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
+                                                   const float4* __restrict__ qcone, int64_t t0, int64_t t1,
+                                                   const FwdRow& r, double& Sr, double& Si) {
+  for (int64_t t = t0; t < t1; ++t) {
+    const double z = qz[t];
+    int mz = kRingFF, mm = kRingFF;
+    QuadThr th{0.0, 0.0, 0.0, 0.0};
+    if (MASK) {
+      bool anyA, allA, anyD, allD, anyB, allB, anyC, allC;
+      classify_pair(qcone[2 * t], r.cl, r.act, anyA, allA, anyD, allD);      // a and its antipode d
+      classify_pair(qcone[2 * t + 1], r.cl, r.act, anyB, allB, anyC, allC);  // b and its antipode c
+      const bool sA = __ballot(anyA) != 0, sB = __ballot(anyB) != 0, sC = __ballot(anyC) != 0,
+                 sD = __ballot(anyD) != 0;
+      const bool wA = __ballot(!allA) == 0, wB = __ballot(!allB) == 0, wC = __ballot(!allC) == 0,
+                 wD = __ballot(!allD) == 0;
+      mz = ring_mode(sA, wA, sB, wB);
+      mm = ring_mode(sC, wC, sD, wD);
+      if (mz == kRingNone && mm == kRingNone) continue;
+      const double za = z * r.mk.za, zb = z * r.mk.zb;
+      th.t1a = r.mk.ta - za;
+      th.t1b = r.mk.tb - zb;
+      th.t2a = r.mk.ta + za;
+      th.t2b = r.mk.tb + zb;
+    }
+    const QuadPack* __restrict__ tq = qp + t * kQuadTile;
+    float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
+#define VISS_QT(A, B)                                                                      \
+  case A * 5 + B:                                                                          \
+    quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, th, Rz, Jz, Rm, Jm);                      \
+    break;
+#define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
+                       VISS_QT(A, kRingTest)
+    switch (mz * 5 + mm) {  // wave-uniform
+      VISS_QT(kRingNone, kRingFF) VISS_QT(kRingNone, kRingFN) VISS_QT(kRingNone, kRingNF)
+      VISS_QT(kRingNone, kRingTest)
+      VISS_QT_ROW(kRingFF) VISS_QT_ROW(kRingFN) VISS_QT_ROW(kRingNF) VISS_QT_ROW(kRingTest)
+      default: break;
+    }
+#undef VISS_QT_ROW
+#undef VISS_QT
+    float sc, cc;
+    sincos_rev(rev_fp32(r.w * z), sc, cc);
+    Sr += (double)(cc * (Rz + Rm)) + (double)(sc * (Jm - Jz));
+    Si += (double)(sc * (Rz - Rm)) + (double)(cc * (Jz + Jm));
+  }
+}
+
 // Applies the phase-tracking term, V e^{-2 pi i c0} = (Sr - i Si)(cos - i sin),
 // and stores as forward_store.
 __device__ __forceinline__ void forward_pair_store(double* __restrict__ V, int64_t k, int64_t K, int herm, double c0,

@@ -73,6 +73,31 @@ __global__ __launch_bounds__(256) void k_forward_pairs(const PairPack* __restric
   if (r.act) forward_pair_store(V, k, K, herm, r.c0, Sr, Si);
 }
 
+// Forward over the equatorial-mirror quad layout (see forward_quad_tiles):
+// grid.y chunks [0, nqchunk) walk quad tiles, the rest walk the pairs and
+// singles left outside the quads.
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_forward_quads(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
+                                                       const float4* __restrict__ qcone, int64_t nqtile, int64_t qpc,
+                                                       int64_t nqchunk, const PairPack* __restrict__ rp,
+                                                       const float4* __restrict__ rcone, int64_t nrtile, int64_t rpc,
+                                                       const double* __restrict__ uvw, const double* __restrict__ xyza,
+                                                       const double* __restrict__ xyzb, int64_t K, double R_moon,
+                                                       double orbit_r, int track, int herm, double* __restrict__ V) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, K, k, k < K, R_moon, orbit_r, track);
+  double Sr = 0.0, Si = 0.0;
+  const int64_t c = blockIdx.y;
+  if (c < nqchunk) {
+    const int64_t t0 = c * qpc, t1 = min(nqtile, t0 + qpc);
+    forward_quad_tiles<MASK>(qp, qz, qcone, t0, t1, r, Sr, Si);
+  } else {
+    const int64_t t0 = (c - nqchunk) * rpc, t1 = min(nrtile, t0 + rpc);
+    forward_pair_tiles<MASK>(rp, rcone, nullptr, t0, t1, r, Sr, Si);
+  }
+  if (r.act) forward_pair_store(V, k, K, herm, r.c0, Sr, Si);
+}
+
 // ------------------------------------------------------------ back-projection
 // Prologue: y'_k = wt_k * y_k [* exp(-2 pi i w_k)], packed with u, v, w.
 __global__ void k_bp_prologue(const double* __restrict__ uvw, const double* __restrict__ xyza,
@@ -398,6 +423,29 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   VISS_REQUIRE(bx <= INT32_MAX, "viss_forward: K too large for one launch");
   dim3 grid((unsigned)bx, (unsigned)cdiv(ntile, tpc));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
+  if (!f64 && sky->use_quads) {
+    // a quad tile holds 64 pixels: 512 per chunk keeps the 32768 pixels of
+    // 256 pair tiles; rest tiles (64 pairs) go 256 per chunk
+    int64_t qpc = env_int("VISS_FWD_QTILES", 0);
+    if (qpc <= 0) {
+      qpc = 2 * kFwdTilesPerChunk;
+      while (qpc > 32 && bx * (cdiv(sky->nqtile, qpc) + cdiv(sky->nrtile, qpc / 2)) < kMinBlocks) qpc /= 2;
+    }
+    qpc = std::max(qpc, cdiv(sky->nqtile, 32768));
+    const int64_t rpc = std::max<int64_t>(qpc / 2, cdiv(sky->nrtile, 32768));
+    const int64_t nqc = cdiv(sky->nqtile, qpc), nrc = cdiv(sky->nrtile, rpc);
+    dim3 qgrid((unsigned)bx, (unsigned)(nqc + nrc));
+    if (mask)
+      k_forward_quads<true><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, qpc, nqc, sky->rpack,
+                                                  sky->rcone, sky->nrtile, rpc, uvw, xyza, xyzb, K, R, orb, track,
+                                                  herm, V);
+    else
+      k_forward_quads<false><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, qpc, nqc,
+                                                   sky->rpack, sky->rcone, sky->nrtile, rpc, uvw, xyza, xyzb, K, R,
+                                                   orb, track, herm, V);
+    VISS_LAUNCHED(k_forward_quads);
+    return VISS_OK;
+  }
   if (pairs) {
     if (mask)
       k_forward_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, ntile, tpc, uvw, xyza, xyzb, K, R, orb, track,

@@ -403,6 +403,344 @@ static int sky_build_pairs(viss_sky* sky, const double* lmn, int64_t P, hipStrea
   return rc;
 }
 
+// ------------------------------------------------ equatorial-mirror quads
+// HEALPix is also symmetric under z -> -z, ring by ring.  Two antipodal pairs
+// whose a pixels (upper hemisphere) are (x, y, z) and (-x, -y, z) form a quad
+// {a, b, c, d} = {(x, y, z), (-x, -y, z), (x, y, -z), (-x, -y, -z)}.  They are
+// found like the pairs: a Morton key of the a pixel's direction with (x, y)
+// replaced by its upper-half-plane representative; a run of exactly two equal
+// keys, opposite half-planes, z and (x, y) agreeing to 1e-12 is a quad.
+
+// Upper half-plane in (x, y), written with selects (see upper_half).
+__device__ inline bool xy_upper(double l, double m) {
+  return fabs(m) > kHemiEps ? (m > 0.0) : (l > 0.0);
+}
+
+__global__ void k_quad_keys(const PairPack* __restrict__ pp, int64_t npair, uint64_t* __restrict__ key,
+                            int32_t* __restrict__ val) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < npair; j += (int64_t)gridDim.x * blockDim.x) {
+    const double l = pp[j].l, m = pp[j].m, n = pp[j].n;
+    const double sg = xy_upper(l, m) ? 1.0 : -1.0;
+    const uint64_t k = spread3_21(quant21(sg * l)) | (spread3_21(quant21(sg * m)) << 1) |
+                       (spread3_21(quant21(n)) << 2);
+    key[j] = n > kHemiEps ? k : ~0ull;  // equator pairs never form quads
+    val[j] = (int32_t)j;
+  }
+}
+
+// Sorted position i starts a quad (pairs val[i], val[i+1]).
+__device__ inline bool quad_starts(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
+                                   const PairPack* __restrict__ pp, int64_t npair, int64_t i) {
+  if (i < 0 || i + 1 >= npair) return false;
+  const uint64_t k = key[i];
+  const bool before = i > 0 ? key[i - 1] == k : false;
+  const bool after = i + 2 < npair ? key[i + 2] == k : false;
+  const PairPack p = pp[val[i]], q = pp[val[i + 1]];
+  const bool up_p = xy_upper(p.l, p.m), up_q = xy_upper(q.l, q.m);
+  // z agrees to 1e-12, not bitwise: the second pair's a pixel is the antipode
+  // of a south-ring pixel, whose cos(pi - theta) can differ from -cos(theta)
+  // by an ulp.  Tiles take the first pair's z (a north-ring pixel, the same
+  // bits along a ring).
+  return (k != ~0ull) & (key[i + 1] == k) & !before & !after & (up_p != up_q) & (fabs(p.n - q.n) <= kPairTol) &
+         (fabs(p.l + q.l) <= kPairTol) & (fabs(p.m + q.m) <= kPairTol);
+}
+
+__global__ void k_quad_flags(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
+                             const PairPack* __restrict__ pp, int64_t npair, uint8_t* __restrict__ is_quad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npair; i += (int64_t)gridDim.x * blockDim.x)
+    is_quad[i] = quad_starts(key, val, pp, npair, i) ? 1 : 0;
+}
+
+// Quad q (sorted position start[q]): members P1 (a pixel in the upper
+// half-plane) and P2; sort keys azimuth then z (both >= 0, so their IEEE bits
+// order like the values).
+__global__ void k_quad_members(const int32_t* __restrict__ start, const int32_t* __restrict__ val,
+                               const PairPack* __restrict__ pp, int64_t nquad, int2* __restrict__ mem,
+                               uint64_t* __restrict__ phi_key, uint64_t* __restrict__ z_key,
+                               int32_t* __restrict__ qid, uint8_t* __restrict__ in_quad) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nquad; q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t i = start[q], j0 = val[i], j1 = val[i + 1];
+    const bool first = xy_upper(pp[j0].l, pp[j0].m);
+    const int32_t p1 = first ? j0 : j1, p2 = first ? j1 : j0;
+    mem[q] = make_int2(p1, p2);
+    const double phi = atan2(pp[p1].m, pp[p1].l) + 0.0;  // [0, pi]; + 0.0 maps -0 to +0
+    phi_key[q] = __double_as_longlong(phi < 0.0 ? 0.0 : phi);
+    z_key[q] = __double_as_longlong(pp[p1].n);
+    qid[q] = (int32_t)q;
+    in_quad[p1] = 1;
+    in_quad[p2] = 1;
+  }
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const int32_t* __restrict__ idx, int64_t n,
+                             uint64_t* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
+__global__ void k_new_ring(const uint64_t* __restrict__ zs, int64_t n, int32_t* __restrict__ flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = (i == 0 || zs[i] != zs[i - 1]) ? 1 : 0;
+}
+
+// Tiles of ring r: ceil(count / kQuadTile); rings beyond nring get none.
+__global__ void k_ring_tiles(const int32_t* __restrict__ rstart, const int32_t* __restrict__ nring_d, int64_t n,
+                             int32_t* __restrict__ ntiles) {
+  const int32_t nring = *nring_d;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t end = r + 1 < nring ? rstart[r + 1] : n;
+    ntiles[r] = r < nring ? (int32_t)((end - rstart[r] + kQuadTile - 1) / kQuadTile) : 0;
+  }
+}
+
+__global__ void k_quad_clear(QuadPack* __restrict__ qp, int4* __restrict__ qi, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    QuadPack z;
+    z.x = z.y = 0.0;
+    z.Az = z.Dz = z.Am = z.Dm = z.Ia = z.Ib = z.Ic = z.Id = 0.f;
+    qp[i] = z;
+    qi[i] = make_int4(-1, -1, -1, -1);
+  }
+}
+
+// Sorted quad i (ring ring_of[i], position i - rstart) -> its tile slot.
+__global__ void k_quad_pack(const int32_t* __restrict__ order, const int2* __restrict__ mem,
+                            const int32_t* __restrict__ ring_incl, const int32_t* __restrict__ rstart,
+                            const int32_t* __restrict__ toff, const PairPack* __restrict__ pp,
+                            const int2* __restrict__ pidx, int64_t nquad, QuadPack* __restrict__ qp,
+                            int4* __restrict__ qi, double* __restrict__ qz) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nquad; i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 m = mem[order[i]];
+    const int32_t r = ring_incl[i] - 1;
+    const int64_t pos = i - rstart[r];
+    const int64_t t = toff[r] + pos / kQuadTile, slot = t * kQuadTile + pos % kQuadTile;
+    const PairPack p1 = pp[m.x];
+    qp[slot].x = p1.l;
+    qp[slot].y = p1.m;
+    const int2 i1 = pidx[m.x], i2 = pidx[m.y];
+    qi[slot] = make_int4(i1.x, i2.x, i2.y, i1.y);  // a = P1.a, b = P2.a, c = P2.b, d = P1.b
+    if (pos % kQuadTile == 0) qz[t] = p1.n;
+  }
+}
+
+// Cones of the a pixels (x, y, z) and b pixels (-x, -y, z) of each tile; the
+// d and c pixels are their antipodes (the negated cones).
+__global__ void k_quad_cones(const QuadPack* __restrict__ qp, const int4* __restrict__ qi,
+                             const double* __restrict__ qz, int64_t ntiles, float4* __restrict__ cone) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    const double z = qz[t];
+    double sx = 0.0, sy = 0.0;
+    int cnt = 0;
+    for (int j = 0; j < kQuadTile; ++j) {
+      const bool v = qi[t * kQuadTile + j].x >= 0;
+      sx += v ? qp[t * kQuadTile + j].x : 0.0;
+      sy += v ? qp[t * kQuadTile + j].y : 0.0;
+      cnt += v ? 1 : 0;
+    }
+    for (int side = 0; side < 2; ++side) {
+      const double sg = side == 0 ? 1.0 : -1.0;
+      const double cx0 = sg * sx, cy0 = sg * sy, cz0 = cnt * z;
+      const double nrm = sqrt(cx0 * cx0 + cy0 * cy0 + cz0 * cz0);
+      double cx = 0.0, cy = 0.0, cz = 1.0, chord = 2.0;
+      if (cnt > 0 && nrm > 1e-6 * cnt) {
+        cx = cx0 / nrm;
+        cy = cy0 / nrm;
+        cz = cz0 / nrm;
+        double mx = 0.0;
+        for (int j = 0; j < kQuadTile; ++j) {
+          const bool v = qi[t * kQuadTile + j].x >= 0;
+          const double dx = sg * qp[t * kQuadTile + j].x - cx, dy = sg * qp[t * kQuadTile + j].y - cy, dz = z - cz;
+          mx = v ? fmax(mx, sqrt(dx * dx + dy * dy + dz * dz)) : mx;
+        }
+        chord = mx * (1.0 + 1e-6) + 1e-6;
+      }
+      cone[2 * t + side] = make_float4((float)cx, (float)cy, (float)cz, (float)chord);
+    }
+  }
+}
+
+__global__ void k_scatter_quads(QuadPack* __restrict__ qp, const int4* __restrict__ qi, int64_t n,
+                                const double* __restrict__ I) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int4 q = qi[j];
+    const double a = q.x >= 0 ? I[q.x] : 0.0, b = q.y >= 0 ? I[q.y] : 0.0;
+    const double c = q.z >= 0 ? I[q.z] : 0.0, d = q.w >= 0 ? I[q.w] : 0.0;
+    qp[j].Az = (float)(a + b);
+    qp[j].Dz = (float)(a - b);
+    qp[j].Am = (float)(c + d);
+    qp[j].Dm = (float)(c - d);
+    qp[j].Ia = (float)a;
+    qp[j].Ib = (float)b;
+    qp[j].Ic = (float)c;
+    qp[j].Id = (float)d;
+  }
+}
+
+// Rest layout: entry t takes main-layout entry src[t] (pairs not in a quad,
+// then the singles), padding where src is -1.
+__global__ void k_rest_gather(const int32_t* __restrict__ rest_pairs, int64_t nrp, int64_t nrp_pad,
+                              int64_t single0, int64_t nsingle, int64_t n, const PairPack* __restrict__ pp,
+                              const int2* __restrict__ pidx, PairPack* __restrict__ rp, int2* __restrict__ ri) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t src = t < nrp ? (int64_t)rest_pairs[t]
+                                : ((t >= nrp_pad && t < nrp_pad + nsingle) ? single0 + (t - nrp_pad) : -1);
+    PairPack z;
+    z.l = z.m = z.n = 0.0;
+    z.A = z.D = z.Ia = z.Ib = 0.f;
+    rp[t] = src >= 0 ? pp[src] : z;
+    ri[t] = src >= 0 ? pidx[src] : make_int2(-1, -1);
+  }
+}
+
+__global__ void k_rest_flags(const uint8_t* __restrict__ in_quad, int64_t npair, uint8_t* __restrict__ rest) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < npair; j += (int64_t)gridDim.x * blockDim.x)
+    rest[j] = in_quad[j] ? 0 : 1;
+}
+
+// Quad layout on top of the pair layout.  Synchronises twice (quad count,
+// tile count).  Leaves use_quads false when the quads cover under half of
+// the pixels or their tiles are under 2/3 full.
+static int sky_build_quads(viss_sky* sky, hipStream_t s) {
+  const int64_t np = sky->npair;
+  if (!sky->use_pairs || np < 2) return VISS_OK;
+  uint64_t *k_in = nullptr, *k_out = nullptr, *phi_key = nullptr, *z_key = nullptr, *k2 = nullptr;
+  int32_t *v_in = nullptr, *v_out = nullptr, *start = nullptr, *counts = nullptr, *qid = nullptr, *order = nullptr,
+          *order2 = nullptr, *flag = nullptr, *ring_incl = nullptr, *rstart = nullptr, *ntl = nullptr,
+          *toff = nullptr, *rest_pairs = nullptr;
+  int2* mem = nullptr;
+  uint8_t *is_quad = nullptr, *in_quad = nullptr, *rest = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0, b = 0;
+  hipcub::CountingInputIterator<int32_t> iota(0);
+  int rc = VISS_OK;
+  VISS_HIP(hipMallocAsync((void**)&k_in, sizeof(uint64_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&k_out, sizeof(uint64_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&phi_key, sizeof(uint64_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&z_key, sizeof(uint64_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&k2, sizeof(uint64_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&v_in, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&v_out, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&start, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&counts, sizeof(int32_t) * 4, s));
+  VISS_HIP(hipMallocAsync((void**)&qid, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&order, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&order2, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&flag, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&ring_incl, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&rstart, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&ntl, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&toff, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&rest_pairs, sizeof(int32_t) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&mem, sizeof(int2) * np, s));
+  VISS_HIP(hipMallocAsync((void**)&is_quad, np, s));
+  VISS_HIP(hipMallocAsync((void**)&in_quad, np, s));
+  VISS_HIP(hipMallocAsync((void**)&rest, np, s));
+  // scratch large enough for every primitive below
+  {
+    size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0;
+    VISS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, k_in, k_out, v_in, v_out, (int)np, 0, 64, s));
+    VISS_HIP(hipcub::DeviceSelect::Flagged(nullptr, t2, iota, is_quad, start, counts, (int)np, s));
+    VISS_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t3, flag, ring_incl, (int)np, s));
+    VISS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, ntl, toff, (int)np, s));
+    VISS_HIP(hipcub::DeviceSelect::Flagged(nullptr, t5, iota, flag, rstart, counts + 1, (int)np, s));
+    tmp_bytes = std::max(std::max(std::max(t1, t2), std::max(t3, t4)), t5);
+  }
+  VISS_HIP(hipMallocAsync(&tmp, tmp_bytes, s));
+  VISS_HIP(hipMemsetAsync(in_quad, 0, np, s));
+  k_quad_keys<<<grid_for(np, 256), 256, 0, s>>>(sky->ppack, np, k_in, v_in);
+  VISS_LAUNCHED(k_quad_keys);
+  b = tmp_bytes;
+  VISS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, k_in, k_out, v_in, v_out, (int)np, 0, 64, s));
+  k_quad_flags<<<grid_for(np, 256), 256, 0, s>>>(k_out, v_out, sky->ppack, np, is_quad);
+  VISS_LAUNCHED(k_quad_flags);
+  b = tmp_bytes;
+  VISS_HIP(hipcub::DeviceSelect::Flagged(tmp, b, iota, is_quad, start, counts, (int)np, s));
+  int32_t hq = 0;
+  VISS_HIP(hipMemcpyAsync(&hq, counts, sizeof(hq), hipMemcpyDeviceToHost, s));
+  VISS_HIP(hipStreamSynchronize(s));
+  const int64_t nq = hq;
+  sky->nquad = nq;
+  if (nq > 0 && 4 * nq >= sky->P / 2) {
+    k_quad_members<<<grid_for(nq, 256), 256, 0, s>>>(start, v_out, sky->ppack, nq, mem, phi_key, z_key, qid,
+                                                     in_quad);
+    VISS_LAUNCHED(k_quad_members);
+    // azimuth, then z (stable): sorted by (z, azimuth)
+    b = tmp_bytes;
+    VISS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, phi_key, k_out, qid, order2, (int)nq, 0, 64, s));
+    k_gather_u64<<<grid_for(nq, 256), 256, 0, s>>>(z_key, order2, nq, k2);
+    VISS_LAUNCHED(k_gather_u64);
+    b = tmp_bytes;
+    VISS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, k2, k_in, order2, order, (int)nq, 0, 64, s));
+    k_new_ring<<<grid_for(nq, 256), 256, 0, s>>>(k_in, nq, flag);
+    VISS_LAUNCHED(k_new_ring);
+    b = tmp_bytes;
+    VISS_HIP(hipcub::DeviceScan::InclusiveSum(tmp, b, flag, ring_incl, (int)nq, s));
+    b = tmp_bytes;
+    VISS_HIP(hipcub::DeviceSelect::Flagged(tmp, b, iota, flag, rstart, counts + 1, (int)nq, s));
+    k_ring_tiles<<<grid_for(nq, 256), 256, 0, s>>>(rstart, counts + 1, nq, ntl);
+    VISS_LAUNCHED(k_ring_tiles);
+    b = tmp_bytes;
+    VISS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b, ntl, toff, (int)nq, s));
+    int32_t last[2] = {0, 0};
+    VISS_HIP(hipMemcpyAsync(&last[0], toff + nq - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    VISS_HIP(hipMemcpyAsync(&last[1], ntl + nq - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    VISS_HIP(hipStreamSynchronize(s));
+    const int64_t nqt = (int64_t)last[0] + last[1];
+    if (3 * nq >= 2 * kQuadTile * nqt) {
+      sky->nqtile = nqt;
+      const int64_t nslot = nqt * kQuadTile;
+      VISS_HIP(hipMalloc((void**)&sky->qpack, sizeof(QuadPack) * nslot));
+      VISS_HIP(hipMalloc((void**)&sky->qidx, sizeof(int4) * nslot));
+      VISS_HIP(hipMalloc((void**)&sky->qz, sizeof(double) * nqt));
+      VISS_HIP(hipMalloc((void**)&sky->qcone, sizeof(float4) * 2 * nqt));
+      k_quad_clear<<<grid_for(nslot, 256), 256, 0, s>>>(sky->qpack, sky->qidx, nslot);
+      VISS_LAUNCHED(k_quad_clear);
+      k_quad_pack<<<grid_for(nq, 256), 256, 0, s>>>(order, mem, ring_incl, rstart, toff, sky->ppack, sky->pidx, nq,
+                                                    sky->qpack, sky->qidx, sky->qz);
+      VISS_LAUNCHED(k_quad_pack);
+      k_quad_cones<<<grid_for(nqt, 256), 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, nqt, sky->qcone);
+      VISS_LAUNCHED(k_quad_cones);
+      // rest: pairs outside quads, then the singles of the pair layout
+      k_rest_flags<<<grid_for(np, 256), 256, 0, s>>>(in_quad, np, rest);
+      VISS_LAUNCHED(k_rest_flags);
+      b = tmp_bytes;
+      VISS_HIP(hipcub::DeviceSelect::Flagged(tmp, b, iota, rest, rest_pairs, counts + 2, (int)np, s));
+      int32_t hr = 0;
+      VISS_HIP(hipMemcpyAsync(&hr, counts + 2, sizeof(hr), hipMemcpyDeviceToHost, s));
+      VISS_HIP(hipStreamSynchronize(s));
+      const int64_t nrp = hr, nrp_pad = (nrp + 63) / 64 * 64, ns_pad = (sky->nsingle + 63) / 64 * 64;
+      const int64_t single0 = (sky->npair + 63) / 64 * 64;
+      sky->nrtile = (nrp_pad + ns_pad) / 64;
+      if (nrp != np - 2 * nq) {
+        set_error("viss_sky: quad layout lost pairs (%lld quads, %lld rest pairs, %lld pairs)", (long long)nq,
+                  (long long)nrp, (long long)np);
+        rc = VISS_ERR_HIP;
+      } else {
+        if (sky->nrtile > 0) {
+          const int64_t n = sky->nrtile * 64;
+          VISS_HIP(hipMalloc((void**)&sky->rpack, sizeof(PairPack) * n));
+          VISS_HIP(hipMalloc((void**)&sky->ridx, sizeof(int2) * n));
+          VISS_HIP(hipMalloc((void**)&sky->rcone, sizeof(float4) * sky->nrtile));
+          k_rest_gather<<<grid_for(n, 256), 256, 0, s>>>(rest_pairs, nrp, nrp_pad, single0, sky->nsingle, n,
+                                                         sky->ppack, sky->pidx, sky->rpack, sky->ridx);
+          VISS_LAUNCHED(k_rest_gather);
+          k_pair_cones<<<(unsigned)((sky->nrtile + 3) / 4), 256, 0, s>>>(sky->rpack, sky->ridx, sky->nrtile,
+                                                                          sky->rcone);
+          VISS_LAUNCHED(k_pair_cones);
+        }
+        sky->use_quads = true;
+      }
+    }
+  }
+  VISS_HIP(hipFreeAsync(tmp, s));
+  for (void* p : {(void*)k_in, (void*)k_out, (void*)phi_key, (void*)z_key, (void*)k2, (void*)v_in, (void*)v_out,
+                  (void*)start, (void*)counts, (void*)qid, (void*)order, (void*)order2, (void*)flag,
+                  (void*)ring_incl, (void*)rstart, (void*)ntl, (void*)toff, (void*)rest_pairs, (void*)mem,
+                  (void*)is_quad, (void*)in_quad, (void*)rest})
+    VISS_HIP(hipFreeAsync(p, s));
+  return rc;
+}
+
 int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
   VISS_REQUIRE(P > 0 && P <= (int64_t)INT32_MAX - kPadTo, "viss_sky: pixel count %lld out of range", (long long)P);
   VISS_HIP(hipGetDevice(&sky->device));
@@ -444,7 +782,9 @@ int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
   VISS_HIP(hipFreeAsync(k_out, s));
   VISS_HIP(hipFreeAsync(v_in, s));
   VISS_HIP(hipFreeAsync(v_out, s));
-  return sky_build_pairs(sky, lmn, P, s);
+  const int rc = sky_build_pairs(sky, lmn, P, s);
+  if (rc != VISS_OK) return rc;
+  return sky_build_quads(sky, s);
 }
 
 void sky_free(viss_sky* sky) {
@@ -460,6 +800,16 @@ void sky_free(viss_sky* sky) {
   sky->ppack = nullptr;
   sky->pidx = nullptr;
   sky->pcone = nullptr;
+  for (void* p : {(void*)sky->qpack, (void*)sky->qidx, (void*)sky->qz, (void*)sky->qcone, (void*)sky->rpack,
+                  (void*)sky->ridx, (void*)sky->rcone})
+    if (p) (void)hipFree(p);
+  sky->qpack = nullptr;
+  sky->qidx = nullptr;
+  sky->qz = nullptr;
+  sky->qcone = nullptr;
+  sky->rpack = nullptr;
+  sky->ridx = nullptr;
+  sky->rcone = nullptr;
   sky->ws.release();
   sky->pix = nullptr;
   sky->I64 = nullptr;
@@ -473,6 +823,15 @@ int sky_set_brightness(viss_sky* sky, const double* I, bool want64, hipStream_t 
   if (sky->nptile > 0) {
     k_scatter_pairs<<<grid_for(sky->nptile * 64, 256), 256, 0, s>>>(sky->ppack, sky->pidx, sky->nptile * 64, I);
     VISS_LAUNCHED(k_scatter_pairs);
+  }
+  if (sky->use_quads) {
+    const int64_t nslot = sky->nqtile * kQuadTile;
+    k_scatter_quads<<<grid_for(nslot, 256), 256, 0, s>>>(sky->qpack, sky->qidx, nslot, I);
+    VISS_LAUNCHED(k_scatter_quads);
+    if (sky->nrtile > 0) {
+      k_scatter_pairs<<<grid_for(sky->nrtile * 64, 256), 256, 0, s>>>(sky->rpack, sky->ridx, sky->nrtile * 64, I);
+      VISS_LAUNCHED(k_scatter_pairs);
+    }
   }
   return VISS_OK;
 }
@@ -531,6 +890,14 @@ int viss_sky_pairs(const viss_sky* sky, int64_t* npair, int64_t* nsingle, int* u
   if (npair) *npair = sky->npair;
   if (nsingle) *nsingle = sky->nsingle;
   if (uses_pairs) *uses_pairs = sky->use_pairs ? 1 : 0;
+  return VISS_OK;
+}
+
+int viss_sky_quads(const viss_sky* sky, int64_t* nquad, int64_t* nquad_tiles, int* uses_quads) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_quads: sky is NULL");
+  if (nquad) *nquad = sky->nquad;
+  if (nquad_tiles) *nquad_tiles = sky->nqtile;
+  if (uses_quads) *uses_quads = sky->use_quads ? 1 : 0;
   return VISS_OK;
 }
 

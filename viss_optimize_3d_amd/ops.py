@@ -100,11 +100,17 @@ class Sky:
 
     def pairs(self):
         """Antipodal pairs and unpaired pixels found at creation, and whether the
-        mixed per-row kernels use the pairs."""
+        mixed per-row kernels use the pairs; equatorial-mirror quads, their
+        16-quad tiles, and whether the mixed per-row forward uses them."""
         n_pair, n_single, uses = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
         check("viss_sky_pairs", _lib.load().viss_sky_pairs(self._h, ctypes.byref(n_pair), ctypes.byref(n_single),
                                                            ctypes.byref(uses)))
-        return {"pairs": n_pair.value, "singles": n_single.value, "uses_pairs": bool(uses.value)}
+        out = {"pairs": n_pair.value, "singles": n_single.value, "uses_pairs": bool(uses.value)}
+        n_quad, n_qtile, uses_q = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        check("viss_sky_quads", _lib.load().viss_sky_quads(self._h, ctypes.byref(n_quad), ctypes.byref(n_qtile),
+                                                           ctypes.byref(uses_q)))
+        out.update({"quads": n_quad.value, "quad_tiles": n_qtile.value, "uses_quads": bool(uses_q.value)})
+        return out
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
