@@ -165,8 +165,9 @@ def test_quad_counts(ops):
 
 @pytest.mark.parametrize("kind", ["rings", "rings-minus"])
 def test_quad_layouts_against_oracle(ops, kind):
-    """Forward over quad skies that are not HEALPix (rings of random azimuths;
-    with 5% of the pixels removed: quads, leftover pairs and singles)."""
+    """Forward and back-projection over quad skies that are not HEALPix (rings
+    of random azimuths; with 5% of the pixels removed: quads, leftover pairs
+    and singles)."""
     rng = np.random.default_rng(11 if kind == "rings" else 12)
     lmn = ring_sky(rng)
     if kind == "rings-minus":
@@ -186,5 +187,13 @@ def test_quad_layouts_against_oracle(ops, kind):
         Vo = oracle.forward(l, m, n, I, uvw, xa, xb, flags=flags, R_moon=R)
         V = sky.forward(t["I"], t["uvw"], t["xa"], t["xb"], mask=mask, phase_track=track, hermitian=herm, R_moon=R)
         rv = rel_l2(V.cpu().numpy(), Vo)
-        print(f"{kind} mask={mask} track={track} herm={herm}: forward {rv:.2e}", flush=True)
-        assert rv <= 1e-5
+        K = uvw.shape[1]
+        y = rng.standard_normal(K) + 1j * rng.standard_normal(K)
+        wt = rng.uniform(0.5, 1.5, K)
+        bflags = flags & ~oracle.HERMITIAN
+        xo = oracle.backproject(l, m, n, uvw, y, wt, xa, xb, flags=bflags, R_moon=R)
+        x = sky.backproject(t["uvw"], torch.from_numpy(y).cuda(), torch.from_numpy(wt).cuda(), t["xa"], t["xb"],
+                            mask=mask, phase_track=track, R_moon=R)
+        rx = rel_l2(x.cpu().numpy(), xo)
+        print(f"{kind} mask={mask} track={track} herm={herm}: forward {rv:.2e}, back-projection {rx:.2e}", flush=True)
+        assert rv <= 1e-5 and rx <= 1e-5

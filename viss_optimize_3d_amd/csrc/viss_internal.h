@@ -166,6 +166,9 @@ struct viss_sky {
   int4* qidx = nullptr;             // [nqtile * kQuadTile] caller indices of (a, b, c, d); -1 if absent
   double* qz = nullptr;             // [nqtile] z of each tile
   float4* qcone = nullptr;          // [2 nqtile] cones of the a and b pixels of each tile
+  int64_t nqpatch = 0;              // back-projection patches: 4 tiles (64 quads), one per ring of a band
+  int4* qpatch = nullptr;           // [nqpatch] tile of each 16-lane group; -1 if absent
+  float4* qpcone = nullptr;         // [2 nqpatch] cones of the a and b pixels of each patch
   int64_t nrtile = 0;               // 64-entry tiles of the rest layout
   viss::PairPack* rpack = nullptr;  // [nrtile * 64]
   int2* ridx = nullptr;             // [nrtile * 64]

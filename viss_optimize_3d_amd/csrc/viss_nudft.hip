@@ -334,6 +334,157 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
   if (ab.y >= 0) unsafeAtomicAdd(&x[ab.y], herm * acc_b);
 }
 
+// ------------------------------------------------------------ back-projection, equatorial-mirror quads
+// With A = u x + v y, C = w z and y' = y e^{2 pi i C}, y'' = y e^{-2 pi i C}:
+//   a = (x, y, z):    Re(y' e^{+2 pi i A}) = y'r cos A - y'i sin A
+//   b = (-x, -y, z):  Re(y' e^{-2 pi i A}) = y'r cos A + y'i sin A
+//   c = (x, y, -z):   the same with y''     (first form)
+//   d = (-x, -y, -z): the same with y''     (second form)
+// so one sin/cos of A serves the four pixels; rows whose ring shows both
+// pixels whole use P = sum y'r cos, Q = sum y'i sin (a = P - Q, b = P + Q).
+struct QuadBpAcc {
+  float Pz, Qz, Az, Bz, Pm, Qm, Cm, Dm;
+};
+
+template <int MR>
+__device__ __forceinline__ void ring_bp(float yr, float yi, float cs, float sn, bool v1, bool v2, float& P, float& Q,
+                                        float& S1, float& S2) {
+  if (MR == kRingFF) {
+    P = fmaf(yr, cs, P);
+    Q = fmaf(yi, sn, Q);
+  } else if (MR == kRingFN) {
+    S1 = fmaf(yr, cs, fmaf(-yi, sn, S1));
+  } else if (MR == kRingNF) {
+    S2 = fmaf(yr, cs, fmaf(yi, sn, S2));
+  } else if (MR == kRingTest) {
+    const float t1 = fmaf(yr, cs, -yi * sn), t2 = fmaf(yr, cs, yi * sn);
+    S1 += v1 ? t1 : 0.f;
+    S2 += v2 ? t2 : 0.f;
+  }
+}
+
+template <int MZ, int MM>
+__device__ __forceinline__ void quad_bp_rows(uint64_t msk, const VisHot* __restrict__ hot,
+                                             const VisMask* __restrict__ vm, const float4* __restrict__ rot,
+                                             int64_t kb, double qx, double qy, double zl, QuadBpAcc& acc) {
+  while (msk) {
+    const int j = __builtin_ctzll(msk);
+    msk &= msk - 1ull;
+    const VisHot h = hot[kb + j];
+    const float4 r = rot[j];  // y' (x, y) and y'' (z, w) of this lane's ring
+    float sn, cs;
+    sincos_rev(rev_fp32(fma(h.u, qx, h.v * qy)), sn, cs);
+    bool va = false, vb = false, vc = false, vd = false;
+    if (MZ == kRingTest || MM == kRingTest) {
+      const VisMask mk = vm[kb + j];
+      const double pa = fma(qx, mk.xa, qy * mk.ya), pb = fma(qx, mk.xb, qy * mk.yb);
+      const double za = zl * mk.za, zb = zl * mk.zb;
+      const double t1a = mk.ta - za, t1b = mk.tb - zb, t2a = mk.ta + za, t2b = mk.tb + zb;
+      va = (pa >= t1a) & (pb >= t1b);
+      vb = (pa <= -t1a) & (pb <= -t1b);
+      vc = (pa >= t2a) & (pb >= t2b);
+      vd = (pa <= -t2a) & (pb <= -t2b);
+    }
+    ring_bp<MZ>(r.x, r.y, cs, sn, va, vb, acc.Pz, acc.Qz, acc.Az, acc.Bz);
+    ring_bp<MM>(r.z, r.w, cs, sn, vc, vd, acc.Pm, acc.Qm, acc.Cm, acc.Dm);
+  }
+}
+
+// One wavefront per patch (4 tiles of 16 quads, one per ring of a band), one
+// quad per lane (lanes 16g..16g+15 take tile patch[g]).  Per 64-row step,
+// lane j loads row j, writes its y rotated by the four rings' factors to the
+// wavefront's LDS slice and classifies the row against the patch's four
+// sides; the rows then run in one loop per (ring z, ring -z) mode pair.
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_backproject_quads(
+    const QuadPack* __restrict__ qp, const int4* __restrict__ qi, const double* __restrict__ qz,
+    const int4* __restrict__ patch, const float4* __restrict__ pcone, int64_t npatch, const VisHot* __restrict__ hot,
+    const VisMask* __restrict__ vm, const VisCls* __restrict__ vc, int64_t K, int64_t k_per_chunk, double herm,
+    double* __restrict__ x) {
+  __shared__ float4 rot_s[4][4][64];  // [wavefront][ring group][row]
+  const int lane = threadIdx.x & 63;
+  const int wv = wave_in_block();
+  const int64_t p = (int64_t)blockIdx.x * 4 + wv;
+  if (p >= npatch) return;  // wave-uniform; the kernel has no block barriers
+  const int g = lane >> 4;
+  const int4 pt = patch[p];
+  const double z0 = pt.x >= 0 ? qz[pt.x] : 0.0, z1 = pt.y >= 0 ? qz[pt.y] : 0.0;
+  const double z2 = pt.z >= 0 ? qz[pt.z] : 0.0, z3 = pt.w >= 0 ? qz[pt.w] : 0.0;
+  const int32_t tile = g == 0 ? pt.x : (g == 1 ? pt.y : (g == 2 ? pt.z : pt.w));
+  const double zl = g == 0 ? z0 : (g == 1 ? z1 : (g == 2 ? z2 : z3));
+  double qx = 0.0, qy = 0.0;
+  int4 idx = make_int4(-1, -1, -1, -1);
+  if (tile >= 0) {
+    const int64_t q = (int64_t)tile * kQuadTile + (lane & 15);
+    qx = qp[q].x;
+    qy = qp[q].y;
+    idx = qi[q];
+  }
+  float4 cnA = make_float4(0.f, 0.f, 1.f, 2.f), cnB = cnA;
+  if (MASK) {
+    cnA = pcone[2 * p];
+    cnB = pcone[2 * p + 1];
+  }
+  const int64_t k0 = (int64_t)blockIdx.y * k_per_chunk;
+  const int64_t k1 = min(K, k0 + k_per_chunk);
+  double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+  float4* __restrict__ rot_w = &rot_s[wv][0][0];
+  for (int64_t kb = k0; kb < k1; kb += 64) {
+    const int nk = (int)min((int64_t)64, k1 - kb);
+    int cat = 0;
+    if (lane < nk) {
+      const VisHot h = hot[kb + lane];
+      const double zg[4] = {z0, z1, z2, z3};
+#pragma unroll
+      for (int gg = 0; gg < 4; ++gg) {
+        float s, c;
+        sincos_rev(rev_fp32(h.w * zg[gg]), s, c);
+        rot_w[gg * 64 + lane] = make_float4(fmaf(h.yr, c, -h.yi * s), fmaf(h.yr, s, h.yi * c),
+                                            fmaf(h.yr, c, h.yi * s), fmaf(h.yi, c, -h.yr * s));
+      }
+      if (MASK) {
+        bool anyA, allA, anyD, allD, anyB, allB, anyC, allC;
+        const VisCls cl = vc[kb + lane];
+        classify_pair(cnA, cl, true, anyA, allA, anyD, allD);
+        classify_pair(cnB, cl, true, anyB, allB, anyC, allC);
+        cat = ring_mode(anyA, allA, anyB, allB) * 5 + ring_mode(anyC, allC, anyD, allD);
+      } else {
+        cat = kRingFF * 5 + kRingFF;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float4* __restrict__ rot = rot_w + g * 64;
+    QuadBpAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#define VISS_QB(A, B) quad_bp_rows<A, B>(__ballot(cat == A * 5 + B), hot, vm, rot, kb, qx, qy, zl, acc);
+    VISS_QB(kRingFF, kRingFF)
+    if (MASK) {
+      VISS_QB(kRingNone, kRingFF) VISS_QB(kRingNone, kRingFN) VISS_QB(kRingNone, kRingNF)
+      VISS_QB(kRingNone, kRingTest)
+      VISS_QB(kRingFF, kRingNone) VISS_QB(kRingFF, kRingFN) VISS_QB(kRingFF, kRingNF) VISS_QB(kRingFF, kRingTest)
+      VISS_QB(kRingFN, kRingNone) VISS_QB(kRingFN, kRingFF) VISS_QB(kRingFN, kRingFN) VISS_QB(kRingFN, kRingNF)
+      VISS_QB(kRingFN, kRingTest)
+      VISS_QB(kRingNF, kRingNone) VISS_QB(kRingNF, kRingFF) VISS_QB(kRingNF, kRingFN) VISS_QB(kRingNF, kRingNF)
+      VISS_QB(kRingNF, kRingTest)
+      VISS_QB(kRingTest, kRingNone) VISS_QB(kRingTest, kRingFF) VISS_QB(kRingTest, kRingFN)
+      VISS_QB(kRingTest, kRingNF) VISS_QB(kRingTest, kRingTest)
+    }
+#undef VISS_QB
+    sa += (double)(acc.Pz - acc.Qz + acc.Az);
+    sb += (double)(acc.Pz + acc.Qz + acc.Bz);
+    sc += (double)(acc.Pm - acc.Qm + acc.Cm);
+    sd += (double)(acc.Pm + acc.Qm + acc.Dm);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (idx.x >= 0) unsafeAtomicAdd(&x[idx.x], herm * sa);
+  if (idx.y >= 0) unsafeAtomicAdd(&x[idx.y], herm * sb);
+  if (idx.z >= 0) unsafeAtomicAdd(&x[idx.z], herm * sc);
+  if (idx.w >= 0) unsafeAtomicAdd(&x[idx.w], herm * sd);
+}
+
 // ------------------------------------------------------------ visible-pair count
 __global__ __launch_bounds__(256) void k_count_visible(const PixPack* __restrict__ pix,
                                                        const float4* __restrict__ cone,
@@ -514,6 +665,28 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
   const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
 #define VISS_BP(M, F) \
   k_backproject<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, kpc, herm, x)
+  if (pairs && sky->use_quads) {
+    // quad patches, then the pairs and singles outside the quads
+    dim3 qgrid((unsigned)cdiv(sky->nqpatch, 4), grid.y);
+    if (mask)
+      k_backproject_quads<true><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,
+                                                      sky->nqpatch, hot, vm, vc, K, kpc, herm, x);
+    else
+      k_backproject_quads<false><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,
+                                                       sky->nqpatch, hot, vm, vc, K, kpc, herm, x);
+    VISS_LAUNCHED(k_backproject_quads);
+    if (sky->nrtile > 0) {
+      dim3 rgrid((unsigned)cdiv(sky->nrtile, 4), grid.y);
+      if (mask)
+        k_backproject_pairs<true><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm, vc,
+                                                        K, kpc, herm, x);
+      else
+        k_backproject_pairs<false><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm,
+                                                         vc, K, kpc, herm, x);
+      VISS_LAUNCHED(k_backproject_pairs);
+    }
+    return VISS_OK;
+  }
   if (pairs) {
     if (mask)
       k_backproject_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc, K,

@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <vector>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -508,7 +509,7 @@ __global__ void k_quad_pack(const int32_t* __restrict__ order, const int2* __res
                             const int32_t* __restrict__ ring_incl, const int32_t* __restrict__ rstart,
                             const int32_t* __restrict__ toff, const PairPack* __restrict__ pp,
                             const int2* __restrict__ pidx, int64_t nquad, QuadPack* __restrict__ qp,
-                            int4* __restrict__ qi, double* __restrict__ qz) {
+                            int4* __restrict__ qi, double* __restrict__ qz, int2* __restrict__ tile_pos) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nquad; i += (int64_t)gridDim.x * blockDim.x) {
     const int2 m = mem[order[i]];
     const int32_t r = ring_incl[i] - 1;
@@ -519,7 +520,10 @@ __global__ void k_quad_pack(const int32_t* __restrict__ order, const int2* __res
     qp[slot].y = p1.m;
     const int2 i1 = pidx[m.x], i2 = pidx[m.y];
     qi[slot] = make_int4(i1.x, i2.x, i2.y, i1.y);  // a = P1.a, b = P2.a, c = P2.b, d = P1.b
-    if (pos % kQuadTile == 0) qz[t] = p1.n;
+    if (pos % kQuadTile == 0) {
+      qz[t] = p1.n;
+      tile_pos[t] = make_int2(r, (int32_t)(pos / kQuadTile));
+    }
   }
 }
 
@@ -555,6 +559,52 @@ __global__ void k_quad_cones(const QuadPack* __restrict__ qp, const int4* __rest
         chord = mx * (1.0 + 1e-6) + 1e-6;
       }
       cone[2 * t + side] = make_float4((float)cx, (float)cy, (float)cz, (float)chord);
+    }
+  }
+}
+
+// Cones of the a pixels and b pixels of each back-projection patch (4 tiles).
+__global__ void k_patch_cones(const QuadPack* __restrict__ qp, const int4* __restrict__ qi,
+                              const double* __restrict__ qz, const int4* __restrict__ patch, int64_t npatch,
+                              float4* __restrict__ cone) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npatch; p += (int64_t)gridDim.x * blockDim.x) {
+    const int4 pt = patch[p];
+    const int32_t tl[4] = {pt.x, pt.y, pt.z, pt.w};
+    for (int side = 0; side < 2; ++side) {
+      const double sg = side == 0 ? 1.0 : -1.0;
+      double sx = 0.0, sy = 0.0, sz = 0.0;
+      int cnt = 0;
+      for (int g = 0; g < 4; ++g) {
+        if (tl[g] < 0) continue;
+        const double z = qz[tl[g]];
+        for (int j = 0; j < kQuadTile; ++j) {
+          const int64_t q = (int64_t)tl[g] * kQuadTile + j;
+          const bool v = qi[q].x >= 0;
+          sx += v ? sg * qp[q].x : 0.0;
+          sy += v ? sg * qp[q].y : 0.0;
+          sz += v ? z : 0.0;
+          cnt += v ? 1 : 0;
+        }
+      }
+      const double nrm = sqrt(sx * sx + sy * sy + sz * sz);
+      double cx = 0.0, cy = 0.0, cz = 1.0, chord = 2.0;
+      if (cnt > 0 && nrm > 1e-6 * cnt) {
+        cx = sx / nrm;
+        cy = sy / nrm;
+        cz = sz / nrm;
+        double mx = 0.0;
+        for (int g = 0; g < 4; ++g) {
+          if (tl[g] < 0) continue;
+          const double z = qz[tl[g]];
+          for (int j = 0; j < kQuadTile; ++j) {
+            const int64_t q = (int64_t)tl[g] * kQuadTile + j;
+            const double dx = sg * qp[q].x - cx, dy = sg * qp[q].y - cy, dz = z - cz;
+            mx = qi[q].x >= 0 ? fmax(mx, sqrt(dx * dx + dy * dy + dz * dz)) : mx;
+          }
+        }
+        chord = mx * (1.0 + 1e-6) + 1e-6;
+      }
+      cone[2 * p + side] = make_float4((float)cx, (float)cy, (float)cz, (float)chord);
     }
   }
 }
@@ -695,11 +745,47 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       VISS_HIP(hipMalloc((void**)&sky->qcone, sizeof(float4) * 2 * nqt));
       k_quad_clear<<<grid_for(nslot, 256), 256, 0, s>>>(sky->qpack, sky->qidx, nslot);
       VISS_LAUNCHED(k_quad_clear);
+      int2* tile_pos = nullptr;
+      VISS_HIP(hipMallocAsync((void**)&tile_pos, sizeof(int2) * nqt, s));
       k_quad_pack<<<grid_for(nq, 256), 256, 0, s>>>(order, mem, ring_incl, rstart, toff, sky->ppack, sky->pidx, nq,
-                                                    sky->qpack, sky->qidx, sky->qz);
+                                                    sky->qpack, sky->qidx, sky->qz, tile_pos);
       VISS_LAUNCHED(k_quad_pack);
       k_quad_cones<<<grid_for(nqt, 256), 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, nqt, sky->qcone);
       VISS_LAUNCHED(k_quad_cones);
+      // Back-projection patches: bands of 4 consecutive rings (in z order);
+      // patch j of a band takes azimuth tile j of each ring, so a wavefront's
+      // 64 quads stay compact.
+      std::vector<int2> hpos((size_t)nqt);
+      VISS_HIP(hipMemcpyAsync(hpos.data(), tile_pos, sizeof(int2) * nqt, hipMemcpyDeviceToHost, s));
+      VISS_HIP(hipStreamSynchronize(s));
+      VISS_HIP(hipFreeAsync(tile_pos, s));
+      std::vector<int4> patches;
+      {
+        int64_t t = 0;
+        while (t < nqt) {
+          const int32_t band = hpos[(size_t)t].x / 4;
+          int64_t e = t;
+          while (e < nqt && hpos[(size_t)e].x / 4 == band) ++e;
+          int32_t nphi = 0;
+          for (int64_t i = t; i < e; ++i) nphi = std::max(nphi, hpos[(size_t)i].y + 1);
+          const size_t p0 = patches.size();
+          patches.resize(p0 + (size_t)nphi, make_int4(-1, -1, -1, -1));
+          for (int64_t i = t; i < e; ++i) {
+            int4& pt = patches[p0 + (size_t)hpos[(size_t)i].y];
+            const int g = hpos[(size_t)i].x % 4;
+            (g == 0 ? pt.x : g == 1 ? pt.y : g == 2 ? pt.z : pt.w) = (int32_t)i;
+          }
+          t = e;
+        }
+      }
+      sky->nqpatch = (int64_t)patches.size();
+      VISS_HIP(hipMalloc((void**)&sky->qpatch, sizeof(int4) * sky->nqpatch));
+      VISS_HIP(hipMalloc((void**)&sky->qpcone, sizeof(float4) * 2 * sky->nqpatch));
+      VISS_HIP(hipMemcpyAsync(sky->qpatch, patches.data(), sizeof(int4) * sky->nqpatch, hipMemcpyHostToDevice, s));
+      k_patch_cones<<<grid_for(sky->nqpatch, 256), 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch,
+                                                                sky->nqpatch, sky->qpcone);
+      VISS_LAUNCHED(k_patch_cones);
+      VISS_HIP(hipStreamSynchronize(s));  // the host patch table goes out of scope
       // rest: pairs outside quads, then the singles of the pair layout
       k_rest_flags<<<grid_for(np, 256), 256, 0, s>>>(in_quad, np, rest);
       VISS_LAUNCHED(k_rest_flags);
@@ -800,9 +886,11 @@ void sky_free(viss_sky* sky) {
   sky->ppack = nullptr;
   sky->pidx = nullptr;
   sky->pcone = nullptr;
-  for (void* p : {(void*)sky->qpack, (void*)sky->qidx, (void*)sky->qz, (void*)sky->qcone, (void*)sky->rpack,
-                  (void*)sky->ridx, (void*)sky->rcone})
+  for (void* p : {(void*)sky->qpack, (void*)sky->qidx, (void*)sky->qz, (void*)sky->qcone, (void*)sky->qpatch,
+                  (void*)sky->qpcone, (void*)sky->rpack, (void*)sky->ridx, (void*)sky->rcone})
     if (p) (void)hipFree(p);
+  sky->qpatch = nullptr;
+  sky->qpcone = nullptr;
   sky->qpack = nullptr;
   sky->qidx = nullptr;
   sky->qz = nullptr;
