@@ -89,7 +89,8 @@ def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
     ff = kib / max(copies_f)
     out = [f"calibration: {calib_mib} MiB copy reads FETCH_SIZE={max(copies_f):.1f} KiB, WRITE_SIZE={max(copies_w):.1f} KiB",
            f"scale factors: fetch x{ff:.4f}, write x{fw:.4f}"]
-    keys = ("k_forward_pairs<true>", "k_backproject_pairs<true>", "k_forward<true, false>",
+    keys = ("k_forward_quads<true>", "k_backproject_quads<true>", "k_forward_pairs<true>",
+            "k_backproject_pairs<true>", "k_forward<true, false>",
             "k_backproject<true, false>", "k_forward_snap_pairs<true, 10>", "k_backproject_snap_pairs<true, 10>",
             "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
             "k_pix2ang", "k_pixel_dirs",
@@ -222,8 +223,10 @@ def main():
                    "scheme": "RING", "seed": wl.seed},
     }
     fwd_avg, adj_avg = float(np.mean(fwd_ms)), float(np.mean(adj_ms))
-    paired = sky.pairs()["uses_pairs"]
-    kf, kb = ("k_forward_pairs", "k_backproject_pairs") if paired else ("k_forward", "k_backproject")
+    layout = sky.pairs()
+    paired, quads = layout["uses_pairs"], layout["uses_quads"]
+    kf, kb = (("k_forward_quads", "k_backproject_quads") if quads else
+              ("k_forward_pairs", "k_backproject_pairs") if paired else ("k_forward", "k_backproject"))
     dom, dom_ms = (kf, fwd_avg) if fwd_avg >= adj_avg else (kb, adj_avg)
     flops = visible * (PAIR_FP32 + PAIR_FP64)
     achieved = flops / (dom_ms * 1e-3)
@@ -231,7 +234,12 @@ def main():
     out["roofline"] = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": committed_traffic(dom, wl.name),
                        "flops_per_pair": {"fp32": PAIR_FP32, "fp64": PAIR_FP64}, "pairs_per_launch": visible}
-    if paired:
+    if quads:
+        out["roofline"]["note"] = ("flops are the frozen per-(visibility, pixel)-pair count; the quad kernels "
+                                   "evaluate one sin/cos per four pixels (x, y, +-z), (-x, -y, +-z) (HEALPix is "
+                                   "symmetric under s -> -s and z -> -z), so they execute fewer flops than "
+                                   "counted and frac can exceed 1")
+    elif paired:
         out["roofline"]["note"] = ("flops are the frozen per-(visibility, pixel)-pair count; the antipodal-pair "
                                    "kernels evaluate one phase and one sin/cos per pixel pair (s, -s), so they "
                                    "execute fewer flops than counted and frac can exceed 1")
@@ -239,7 +247,7 @@ def main():
         "visible_pairs_per_pass": visible, "dense_pairs_per_pass": dense, "visible_fraction": visible / dense,
         "fwd_ms": fwd_avg, "adj_ms": adj_avg, "fwd_pairs_per_s": visible / (fwd_avg * 1e-3),
         "adj_pairs_per_s": visible / (adj_avg * 1e-3), "dense_equiv_pairs_per_s": dense / (wall / args.steps),
-        "sky_build_ms": sky_ms, "antipodal_pairs": sky.pairs(),
+        "sky_build_ms": sky_ms, "sky_layout": layout,
     }
 
     if not args.no_fp64:

@@ -676,13 +676,18 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
                                                        sky->nqpatch, hot, vm, vc, K, kpc, herm, x);
     VISS_LAUNCHED(k_backproject_quads);
     if (sky->nrtile > 0) {
-      dim3 rgrid((unsigned)cdiv(sky->nrtile, 4), grid.y);
+      // few tiles (the equator ring of HEALPix): shorter row chunks, so that
+      // about 16k wavefronts share the work rather than a few long ones
+      int64_t rpc = cdiv(cdiv(K * sky->nrtile, 16384), 64) * 64;
+      rpc = std::min(std::max<int64_t>(rpc, 64), kpc);
+      if (cdiv(K, rpc) > 65535) rpc = cdiv(cdiv(K, 65535), 64) * 64;
+      dim3 rgrid((unsigned)cdiv(sky->nrtile, 4), (unsigned)cdiv(K, rpc));
       if (mask)
         k_backproject_pairs<true><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm, vc,
-                                                        K, kpc, herm, x);
+                                                        K, rpc, herm, x);
       else
         k_backproject_pairs<false><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm,
-                                                         vc, K, kpc, herm, x);
+                                                         vc, K, rpc, herm, x);
       VISS_LAUNCHED(k_backproject_pairs);
     }
     return VISS_OK;
