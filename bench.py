@@ -92,7 +92,8 @@ def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
     keys = ("k_forward_quads<true>", "k_backproject_quads<true>", "k_forward_pairs<true>",
             "k_backproject_pairs<true>", "k_forward<true, false>",
             "k_backproject<true, false>", "k_forward_snap_pairs<true, 10>", "k_backproject_snap_pairs<true, 10>",
-            "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
+            "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_quads<true>",
+            "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
             "k_pix2ang", "k_pixel_dirs",
             "k_bp_prologue", "k_shell_hist", "k_row_weights")
     for key in keys:

@@ -137,6 +137,55 @@ __global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ 
   if (threadIdx.x == 0) count[g] = running;
 }
 
+// Tiles of the quad layout, then the rest tiles, as one index space: two
+// cones per tile (quad tiles: a and b pixels; rest tiles: the a cone twice),
+// each standing for itself and its antipode.
+__global__ void k_batch_cones(const float4* __restrict__ qcone, int64_t nqtile, const float4* __restrict__ rcone,
+                              int64_t nrtile, float4* __restrict__ out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nqtile + nrtile;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const float4 c1 = t < nqtile ? qcone[2 * t] : rcone[t - nqtile];
+    const float4 c2 = t < nqtile ? qcone[2 * t + 1] : rcone[t - nqtile];
+    out[2 * t] = c1;
+    out[2 * t + 1] = c2;
+  }
+}
+
+// Compacted list of the tiles (two cones each, both sides) some satellite of
+// the group may see.  One block per group.
+__global__ __launch_bounds__(256) void k_group_tiles2(const float4* __restrict__ cone2, int64_t ntile,
+                                                      const float4* __restrict__ gcone, const float* __restrict__ gcut,
+                                                      int32_t* __restrict__ tiles, int32_t* __restrict__ count) {
+  __shared__ int wcount[4];
+  const int g = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const float4 gc = gcone[g];
+  const float cut = gcut[g];
+  int32_t* out = tiles + (int64_t)g * ntile;
+  int running = 0;
+  for (int64_t base = 0; base < ntile; base += 256) {
+    const int64_t t = base + threadIdx.x;
+    bool keep = false;
+    if (t < ntile) {
+      for (int h = 0; h < 2; ++h) {
+        const float4 c = cone2[2 * t + h];
+        const float d = fmaf(c.x, gc.x, fmaf(c.y, gc.y, c.z * gc.z));
+        keep |= (d + c.w + gc.w >= cut - kClsEps) | (-d + c.w + gc.w >= cut - kClsEps);
+      }
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) wcount[wv] = __popcll(bal);
+    __syncthreads();
+    int before = running;
+    for (int i = 0; i < wv; ++i) before += wcount[i];
+    const int within = __popcll(bal & ((1ull << lane) - 1ull));
+    if (keep) out[before + within] = (int32_t)t;
+    const int total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+    running += total;
+  }
+  if (threadIdx.x == 0) count[g] = running;
+}
+
 // One launch over the whole work table, one wavefront per item (up to 64
 // rows, one per lane).  The hardware dispatcher is the work queue.  A
 // persistent loop pulling items through an atomic counter measured 2.6x
@@ -184,6 +233,39 @@ __global__ __launch_bounds__(256) void k_forward_batch_pairs(const PairPack* __r
   if (r.act) forward_pair_store(V, k, Ktot, 0, r.c0, Sr, Si);
 }
 
+// The same work table over the quad layout (mixed precision): list entries
+// below nqtile are 16-quad tiles, the others tiles of the rest layout.
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_forward_batch_quads(const QuadPack* __restrict__ qp,
+                                                             const double* __restrict__ qz,
+                                                             const float4* __restrict__ qcone, int64_t nqtile,
+                                                             const PairPack* __restrict__ rp,
+                                                             const float4* __restrict__ rcone,
+                                                             const int32_t* __restrict__ tiles, int64_t ntile,
+                                                             const WorkItem* __restrict__ items, int64_t n_items,
+                                                             const double* __restrict__ uvw,
+                                                             const double* __restrict__ xyza,
+                                                             const double* __restrict__ xyzb, int64_t Ktot,
+                                                             double R_moon, double orbit_r, int track,
+                                                             double* __restrict__ V) {
+  const int lane = threadIdx.x & 63;
+  const int64_t it = (int64_t)blockIdx.x * 4 + wave_in_block();
+  if (it >= n_items) return;  // wave-uniform
+  const WorkItem w = items[it];
+  const int64_t k = w.row0 + lane;
+  const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, lane < w.nrows, R_moon, orbit_r, track);
+  const int32_t* __restrict__ list = tiles + (int64_t)w.g * ntile;
+  double Sr = 0.0, Si = 0.0;
+  for (int64_t j = w.j0; j < w.j1; ++j) {
+    const int64_t t = list[j];  // wave-uniform
+    if (t < nqtile)
+      forward_quad_tiles<MASK>(qp, qz, qcone, t, t + 1, r, Sr, Si);
+    else
+      forward_pair_tiles<MASK>(rp, rcone, nullptr, t - nqtile, t - nqtile + 1, r, Sr, Si);
+  }
+  if (r.act) forward_pair_store(V, k, Ktot, 0, r.c0, Sr, Si);
+}
+
 }  // namespace viss
 
 using namespace viss;
@@ -197,6 +279,8 @@ struct viss_batch {
   double R = 0, orb = 0;
   int flags = 0;
   bool pairs = false;   // mixed precision over a paired sky: tiles of 64 antipodal pairs
+  bool quads = false;   // mixed precision over the quad layout: quad tiles, then rest tiles
+  float4* tcone2 = nullptr;  // [2 ntile] cones of the quad layout's tiles (quads)
   int64_t ntile = 0;    // tiles of the layout in use (pairs: sky->nptile, else sky->ntile)
   int64_t* d_offs = nullptr;
   float4* gcone = nullptr;
@@ -290,6 +374,18 @@ static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, d
   const unsigned grid = (unsigned)blocks;
   const int track = (b->flags & VISS_PHASE_TRACK) ? 1 : 0;
   const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
+  if (b->quads) {
+    if (mask)
+      k_forward_batch_quads<true><<<grid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, sky->rpack,
+                                                       sky->rcone, b->tiles, b->ntile, items, n_items, b->uvw, b->xa,
+                                                       b->xb, b->Ktot, b->R, b->orb, track, V);
+    else
+      k_forward_batch_quads<false><<<grid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, sky->rpack,
+                                                        sky->rcone, b->tiles, b->ntile, items, n_items, b->uvw, b->xa,
+                                                        b->xb, b->Ktot, b->R, b->orb, track, V);
+    VISS_LAUNCHED(k_forward_batch_quads);
+    return VISS_OK;
+  }
   if (b->pairs) {
     if (mask)
       k_forward_batch_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pcone, b->tiles, b->ntile, items, n_items,
@@ -320,6 +416,7 @@ static void batch_free(viss_batch* b) {
   if (b->gcone) (void)hipFree(b->gcone);
   if (b->gcut) (void)hipFree(b->gcut);
   if (b->tiles) (void)hipFree(b->tiles);
+  if (b->tcone2) (void)hipFree(b->tcone2);
   if (b->d_count) (void)hipFree(b->d_count);
   if (b->items) (void)hipFree(b->items);
 }
@@ -331,7 +428,14 @@ static int batch_build(viss_batch* b, hipStream_t s) {
   VISS_HIP(hipMalloc((void**)&b->gcone, sizeof(float4) * G));
   VISS_HIP(hipMalloc((void**)&b->gcut, sizeof(float) * G));
   b->pairs = !(b->flags & VISS_PREC_FP64) && sky->use_pairs;
-  b->ntile = b->pairs ? sky->nptile : sky->ntile;
+  b->quads = b->pairs && sky->use_quads;
+  b->ntile = b->quads ? sky->nqtile + sky->nrtile : (b->pairs ? sky->nptile : sky->ntile);
+  if (b->quads) {
+    VISS_HIP(hipMalloc((void**)&b->tcone2, sizeof(float4) * 2 * b->ntile));
+    k_batch_cones<<<(unsigned)std::min<int64_t>((b->ntile + 255) / 256, 65536), 256, 0, s>>>(
+        sky->qcone, sky->nqtile, sky->rcone, sky->nrtile, b->tcone2);
+    VISS_LAUNCHED(k_batch_cones);
+  }
   VISS_HIP(hipMalloc((void**)&b->tiles, sizeof(int32_t) * G * b->ntile));
   VISS_HIP(hipMalloc((void**)&b->d_count, sizeof(int32_t) * G));
   VISS_HIP(hipMemcpyAsync(b->d_offs, b->offs.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, s));
@@ -340,9 +444,14 @@ static int batch_build(viss_batch* b, hipStream_t s) {
   if (b->flags & VISS_MASK) {
     k_group_bounds<<<(unsigned)G, 256, 0, s>>>(b->xa, b->xb, b->Ktot, b->d_offs, b->R, b->orb, b->gcone, b->gcut);
     VISS_LAUNCHED(k_group_bounds);
-    k_group_tiles<<<(unsigned)G, 256, 0, s>>>(b->pairs ? sky->pcone : sky->cone, b->ntile, b->pairs ? 1 : 0, b->gcone,
-                                              b->gcut, b->tiles, b->d_count);
-    VISS_LAUNCHED(k_group_tiles);
+    if (b->quads) {
+      k_group_tiles2<<<(unsigned)G, 256, 0, s>>>(b->tcone2, b->ntile, b->gcone, b->gcut, b->tiles, b->d_count);
+      VISS_LAUNCHED(k_group_tiles2);
+    } else {
+      k_group_tiles<<<(unsigned)G, 256, 0, s>>>(b->pairs ? sky->pcone : sky->cone, b->ntile, b->pairs ? 1 : 0,
+                                                b->gcone, b->gcut, b->tiles, b->d_count);
+      VISS_LAUNCHED(k_group_tiles);
+    }
     VISS_HIP(hipMemcpyAsync(b->ntiles_g.data(), b->d_count, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
   } else {
     all.resize(b->ntile);
