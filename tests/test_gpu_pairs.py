@@ -161,15 +161,19 @@ def test_quad_counts(ops):
     assert got["quads"] == 20 * 48 and got["quad_tiles"] == 20 * 3 and got["uses_quads"]
     got = ops.Sky(torch.from_numpy(sky_of("exact-pm", rng)).cuda()).pairs()
     assert got["quads"] == 0 and not got["uses_quads"]
+    # quads in tiles under 2/3 full (3 per ring): the pair kernels run instead
+    got = ops.Sky(torch.from_numpy(ring_sky(rng, nring=40, per_ring=3)).cuda()).pairs()
+    print(f"sparse rings: {got}", flush=True)
+    assert got["quads"] == 120 and got["uses_pairs"] and not got["uses_quads"]
 
 
-@pytest.mark.parametrize("kind", ["rings", "rings-minus"])
+@pytest.mark.parametrize("kind", ["rings", "rings-minus", "rings-sparse"])
 def test_quad_layouts_against_oracle(ops, kind):
     """Forward and back-projection over quad skies that are not HEALPix (rings
     of random azimuths; with 5% of the pixels removed: quads, leftover pairs
-    and singles)."""
-    rng = np.random.default_rng(11 if kind == "rings" else 12)
-    lmn = ring_sky(rng)
+    and singles; sparse rings: quads too few per tile, so the pair kernels)."""
+    rng = np.random.default_rng({"rings": 11, "rings-minus": 12, "rings-sparse": 13}[kind])
+    lmn = ring_sky(rng, nring=40, per_ring=3) if kind == "rings-sparse" else ring_sky(rng)
     if kind == "rings-minus":
         keep = np.sort(rng.choice(lmn.shape[1], int(lmn.shape[1] * 0.95), replace=False))
         lmn = lmn[:, keep]
