@@ -32,6 +32,12 @@
 //  * VISS_PREC_FP64: same structure, FP64 accumulation; the forward uses
 //    FP64 sincospi, the back-projection one FP64 cospi per (row, pixel) on
 //    the polar form of y (x is real).
+//  * sky symmetry (mixed precision): skies whose pixels pair with their
+//    antipodes run the pair kernels (one phase and sin/cos per (s, -s));
+//    skies whose pairs also mirror in z ring by ring (HEALPix) run the quad
+//    kernels (one sin/cos per {(x, y, +-z), (-x, -y, +-z)}, the ring factor
+//    exp(2 pi i w z) per tile in the forward and per 64 rows in the
+//    back-projection).  The per-pixel kernels above serve every other sky.
 #include <algorithm>
 #include <cstdlib>
 
