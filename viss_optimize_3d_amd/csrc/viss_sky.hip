@@ -333,6 +333,32 @@ int check_device(const viss_sky* sky) {
   return VISS_OK;
 }
 
+// Temporary device buffers of a build step, freed on the stream when the step
+// returns (early error returns included).
+class StreamBufs {
+ public:
+  explicit StreamBufs(hipStream_t s) : s_(s) {}
+  StreamBufs(const StreamBufs&) = delete;
+  StreamBufs& operator=(const StreamBufs&) = delete;
+  ~StreamBufs() {
+    for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
+  }
+  template <class T>
+  hipError_t alloc(T** p, size_t bytes) {
+    void* q = nullptr;
+    const hipError_t e = hipMallocAsync(&q, bytes, s_);
+    if (e == hipSuccess) {
+      ptrs_.push_back(q);
+      *p = static_cast<T*>(q);
+    }
+    return e;
+  }
+
+ private:
+  hipStream_t s_;
+  std::vector<void*> ptrs_;
+};
+
 // Antipodal-pair layout (see k_pair_keys).  Synchronises once, to size the
 // arrays from the pair count.
 static int sky_build_pairs(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
@@ -341,22 +367,23 @@ static int sky_build_pairs(viss_sky* sky, const double* lmn, int64_t P, hipStrea
   uint8_t *is_pair = nullptr, *is_single = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0, sel_bytes = 0;
-  VISS_HIP(hipMallocAsync((void**)&k_in, sizeof(uint64_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&k_out, sizeof(uint64_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&v_in, sizeof(int32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&v_out, sizeof(int32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&pair_pos, sizeof(int32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&single_pos, sizeof(int32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&counts, sizeof(int32_t) * 2, s));
-  VISS_HIP(hipMallocAsync((void**)&is_pair, P, s));
-  VISS_HIP(hipMallocAsync((void**)&is_single, P, s));
+  StreamBufs bufs(s);
+  VISS_HIP(bufs.alloc(&k_in, sizeof(uint64_t) * P));
+  VISS_HIP(bufs.alloc(&k_out, sizeof(uint64_t) * P));
+  VISS_HIP(bufs.alloc(&v_in, sizeof(int32_t) * P));
+  VISS_HIP(bufs.alloc(&v_out, sizeof(int32_t) * P));
+  VISS_HIP(bufs.alloc(&pair_pos, sizeof(int32_t) * P));
+  VISS_HIP(bufs.alloc(&single_pos, sizeof(int32_t) * P));
+  VISS_HIP(bufs.alloc(&counts, sizeof(int32_t) * 2));
+  VISS_HIP(bufs.alloc(&is_pair, P));
+  VISS_HIP(bufs.alloc(&is_single, P));
   k_pair_keys<<<grid_for(P, 256), 256, 0, s>>>(lmn, P, k_in, v_in);
   VISS_LAUNCHED(k_pair_keys);
   VISS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k_in, k_out, v_in, v_out, (int)P, 0, 63, s));
   hipcub::CountingInputIterator<int32_t> iota(0);
   VISS_HIP(hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, iota, is_pair, pair_pos, counts, (int)P, s));
   tmp_bytes = std::max(tmp_bytes, sel_bytes);
-  VISS_HIP(hipMallocAsync(&tmp, tmp_bytes, s));
+  VISS_HIP(bufs.alloc(&tmp, tmp_bytes));
   VISS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k_in, k_out, v_in, v_out, (int)P, 0, 63, s));
   k_pair_flags<<<grid_for(P, 256), 256, 0, s>>>(k_out, v_out, lmn, P, is_pair, is_single);
   VISS_LAUNCHED(k_pair_flags);
@@ -391,16 +418,6 @@ static int sky_build_pairs(viss_sky* sky, const double* lmn, int64_t P, hipStrea
       VISS_LAUNCHED(k_pair_cones);
     }
   }
-  VISS_HIP(hipFreeAsync(tmp, s));
-  VISS_HIP(hipFreeAsync(k_in, s));
-  VISS_HIP(hipFreeAsync(k_out, s));
-  VISS_HIP(hipFreeAsync(v_in, s));
-  VISS_HIP(hipFreeAsync(v_out, s));
-  VISS_HIP(hipFreeAsync(pair_pos, s));
-  VISS_HIP(hipFreeAsync(single_pos, s));
-  VISS_HIP(hipFreeAsync(counts, s));
-  VISS_HIP(hipFreeAsync(is_pair, s));
-  VISS_HIP(hipFreeAsync(is_single, s));
   return rc;
 }
 
@@ -663,28 +680,29 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
   size_t tmp_bytes = 0, b = 0;
   hipcub::CountingInputIterator<int32_t> iota(0);
   int rc = VISS_OK;
-  VISS_HIP(hipMallocAsync((void**)&k_in, sizeof(uint64_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&k_out, sizeof(uint64_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&phi_key, sizeof(uint64_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&z_key, sizeof(uint64_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&k2, sizeof(uint64_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&v_in, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&v_out, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&start, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&counts, sizeof(int32_t) * 4, s));
-  VISS_HIP(hipMallocAsync((void**)&qid, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&order, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&order2, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&flag, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&ring_incl, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&rstart, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&ntl, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&toff, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&rest_pairs, sizeof(int32_t) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&mem, sizeof(int2) * np, s));
-  VISS_HIP(hipMallocAsync((void**)&is_quad, np, s));
-  VISS_HIP(hipMallocAsync((void**)&in_quad, np, s));
-  VISS_HIP(hipMallocAsync((void**)&rest, np, s));
+  StreamBufs bufs(s);
+  VISS_HIP(bufs.alloc(&k_in, sizeof(uint64_t) * np));
+  VISS_HIP(bufs.alloc(&k_out, sizeof(uint64_t) * np));
+  VISS_HIP(bufs.alloc(&phi_key, sizeof(uint64_t) * np));
+  VISS_HIP(bufs.alloc(&z_key, sizeof(uint64_t) * np));
+  VISS_HIP(bufs.alloc(&k2, sizeof(uint64_t) * np));
+  VISS_HIP(bufs.alloc(&v_in, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&v_out, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&start, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&counts, sizeof(int32_t) * 4));
+  VISS_HIP(bufs.alloc(&qid, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&order, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&order2, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&flag, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&ring_incl, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&rstart, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&ntl, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&toff, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&rest_pairs, sizeof(int32_t) * np));
+  VISS_HIP(bufs.alloc(&mem, sizeof(int2) * np));
+  VISS_HIP(bufs.alloc(&is_quad, np));
+  VISS_HIP(bufs.alloc(&in_quad, np));
+  VISS_HIP(bufs.alloc(&rest, np));
   // scratch large enough for every primitive below
   {
     size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0;
@@ -695,7 +713,7 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
     VISS_HIP(hipcub::DeviceSelect::Flagged(nullptr, t5, iota, flag, rstart, counts + 1, (int)np, s));
     tmp_bytes = std::max(std::max(std::max(t1, t2), std::max(t3, t4)), t5);
   }
-  VISS_HIP(hipMallocAsync(&tmp, tmp_bytes, s));
+  VISS_HIP(bufs.alloc(&tmp, tmp_bytes));
   VISS_HIP(hipMemsetAsync(in_quad, 0, np, s));
   k_quad_keys<<<grid_for(np, 256), 256, 0, s>>>(sky->ppack, np, k_in, v_in);
   VISS_LAUNCHED(k_quad_keys);
@@ -746,7 +764,7 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       k_quad_clear<<<grid_for(nslot, 256), 256, 0, s>>>(sky->qpack, sky->qidx, nslot);
       VISS_LAUNCHED(k_quad_clear);
       int2* tile_pos = nullptr;
-      VISS_HIP(hipMallocAsync((void**)&tile_pos, sizeof(int2) * nqt, s));
+      VISS_HIP(bufs.alloc(&tile_pos, sizeof(int2) * nqt));
       k_quad_pack<<<grid_for(nq, 256), 256, 0, s>>>(order, mem, ring_incl, rstart, toff, sky->ppack, sky->pidx, nq,
                                                     sky->qpack, sky->qidx, sky->qz, tile_pos);
       VISS_LAUNCHED(k_quad_pack);
@@ -758,7 +776,6 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       std::vector<int2> hpos((size_t)nqt);
       VISS_HIP(hipMemcpyAsync(hpos.data(), tile_pos, sizeof(int2) * nqt, hipMemcpyDeviceToHost, s));
       VISS_HIP(hipStreamSynchronize(s));
-      VISS_HIP(hipFreeAsync(tile_pos, s));
       std::vector<int4> patches;
       {
         int64_t t = 0;
@@ -818,12 +835,6 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       }
     }
   }
-  VISS_HIP(hipFreeAsync(tmp, s));
-  for (void* p : {(void*)k_in, (void*)k_out, (void*)phi_key, (void*)z_key, (void*)k2, (void*)v_in, (void*)v_out,
-                  (void*)start, (void*)counts, (void*)qid, (void*)order, (void*)order2, (void*)flag,
-                  (void*)ring_incl, (void*)rstart, (void*)ntl, (void*)toff, (void*)rest_pairs, (void*)mem,
-                  (void*)is_quad, (void*)in_quad, (void*)rest})
-    VISS_HIP(hipFreeAsync(p, s));
   return rc;
 }
 
@@ -840,14 +851,15 @@ int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
   int32_t *v_in = nullptr, *v_out = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
-  VISS_HIP(hipMallocAsync((void**)&k_in, sizeof(uint32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&k_out, sizeof(uint32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&v_in, sizeof(int32_t) * P, s));
-  VISS_HIP(hipMallocAsync((void**)&v_out, sizeof(int32_t) * P, s));
+  StreamBufs bufs(s);
+  VISS_HIP(bufs.alloc(&k_in, sizeof(uint32_t) * P));
+  VISS_HIP(bufs.alloc(&k_out, sizeof(uint32_t) * P));
+  VISS_HIP(bufs.alloc(&v_in, sizeof(int32_t) * P));
+  VISS_HIP(bufs.alloc(&v_out, sizeof(int32_t) * P));
   k_morton_keys<<<grid_for(P, 256), 256, 0, s>>>(lmn, P, k_in, v_in);
   VISS_LAUNCHED(k_morton_keys);
   VISS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k_in, k_out, v_in, v_out, (int)P, 0, 30, s));
-  VISS_HIP(hipMallocAsync(&tmp, tmp_bytes, s));
+  VISS_HIP(bufs.alloc(&tmp, tmp_bytes));
   VISS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k_in, k_out, v_in, v_out, (int)P, 0, 30, s));
 
   VISS_HIP(hipMalloc((void**)&sky->pix, sizeof(PixPack) * Pp));
@@ -863,11 +875,6 @@ int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s) {
   k_tile_cones<<<(unsigned)((sky->nbtile + 3) / 4), 256, 0, s>>>(sky->pix, sky->nbtile, kBpTile / 64, sky->bcone,
                                                                    nullptr);
   VISS_LAUNCHED(k_tile_cones);
-  VISS_HIP(hipFreeAsync(tmp, s));
-  VISS_HIP(hipFreeAsync(k_in, s));
-  VISS_HIP(hipFreeAsync(k_out, s));
-  VISS_HIP(hipFreeAsync(v_in, s));
-  VISS_HIP(hipFreeAsync(v_out, s));
   const int rc = sky_build_pairs(sky, lmn, P, s);
   if (rc != VISS_OK) return rc;
   return sky_build_quads(sky, s);
