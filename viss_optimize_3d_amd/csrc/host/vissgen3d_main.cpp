@@ -35,6 +35,7 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -72,7 +73,7 @@ struct Options {
   double R_moon = 1737.1e3, altitude = 300e3;  // vissgen3d_noblockage.cu:240-241
   bool per_sat_cut = false;
   double bl_max = 100e3;  // :248
-  bool dcf_shift = false, extend_nr = true, fp64 = false, resume = false, binary = false;
+  bool dcf_shift = false, extend_nr = true, fp64 = false, resume = false, binary = false, timing = false;
   int image_mask = -1;  // -1: per variant
   int precision = 0;
   int device = 0;
@@ -84,7 +85,7 @@ int usage() {
           "                 [--sky FILE] [--theta FILE] [--phi FILE] [--freq HZ] [--days N] [--first-day D]\n"
           "                 [--lambda-c] [--R-moon M] [--altitude M] [--per-satellite-cut] [--bl-max M]\n"
           "                 [--dcf-blockage-shift] [--no-extend-nr] [--image-mask on|off] [--fp64]\n"
-          "                 [--precision P] [--binary] [--resume] [--device D]\n");
+          "                 [--precision P] [--binary] [--resume] [--timing] [--device D]\n");
   return 2;
 }
 
@@ -152,6 +153,7 @@ int main(int argc, char** argv) {
     else if (a == "--precision" && (v = next())) o.precision = atoi(v);
     else if (a == "--resume") o.resume = true;
     else if (a == "--binary") o.binary = true;
+    else if (a == "--timing") o.timing = true;
     else if (a == "--device" && (v = next())) o.device = atoi(v);
     else return usage();
   }
@@ -243,6 +245,9 @@ int main(int argc, char** argv) {
       printf("day %d: %s exists, skipped\n", day, out_path.c_str());
       continue;
     }
+    using clk = std::chrono::steady_clock;
+    const auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+    const clk::time_point t_read = clk::now();
     std::vector<double> uvw_r, xa_r, xb_r, bll_r;
     const size_t n_uvw = read_rows(o.in_dir + "uvw" + tag, 3, 3, uvw_r);
     const size_t n_xa = read_rows(o.in_dir + "xyza" + tag, 3, 3, xa_r);
@@ -251,6 +256,8 @@ int main(int argc, char** argv) {
     printf("day %d: uvw_index %zu xyz1_index %zu xyz2_index %zu bll_index %zu\n", day, n_uvw, n_xa, n_xb, n_bll);
     if (n_xa != n_uvw || n_xb != n_uvw || n_bll != n_uvw)
       fprintf(stderr, "vissgen3d: day %d: row counts differ; missing rows read as 0\n", day);
+    const double s_read = since(t_read);
+    const clk::time_point t_gpu = clk::now();
     const size_t rows = n_uvw;
     const size_t amount = rows / 2;  // vissgen3d_noblockage.cu:400 (integer division)
     std::fill(hC.begin(), hC.end(), 0.0);
@@ -342,6 +349,8 @@ int main(int argc, char** argv) {
       HIPCHK(hipMemcpy(hC.data(), d_C, P * sizeof(double), hipMemcpyDeviceToHost));
       printf("day %d: forward %zu rows, imaging %s\n", day, amount, fold ? "folded Hermitian halves" : "all rows");
     }
+    const double s_gpu = since(t_gpu);
+    const clk::time_point t_write = clk::now();
     if (!vissio::write_column(out_path, hC.data(), P, o.precision)) {
       fprintf(stderr, "vissgen3d: cannot write %s\n", out_path.c_str());
       return 1;
@@ -356,6 +365,9 @@ int main(int argc, char** argv) {
       }
       fclose(f);
     }
+    if (o.timing)
+      printf("day %d: read %.3f s (4 text files), host prep + gpu %.3f s, write %.3f s\n", day, s_read, s_gpu,
+             since(t_write));
     printf("day %d: saved %s\n", day, out_path.c_str());
   }
   viss_sky_destroy(sky);
