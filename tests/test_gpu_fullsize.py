@@ -10,7 +10,8 @@ need the oracle to finish the whole problem:
     snapshots, 10 MHz, sigma 50 km): the mixed kernels against the all-FP64
     kernels over every row and pixel (7.1e11 pair evaluations each way), the
     FP64 forward against the oracle on 256 rows, and the adjoint identity
-    <A x, y> = <x, A^H y> of the mixed pair at full size.
+    <A x, y> = <x, A^H y> of the mixed pair at full size;
+  * a 50M-pixel sky (nside 2048): the quad layout at scale, mixed vs FP64.
 
 Tolerances: mixed vs FP64 (GPU or oracle) relative L2 <= 1e-5 (the precision
 target; measured values near 3e-7 are printed); all-FP64 GPU vs the oracle
@@ -117,3 +118,34 @@ def test_precision_workload_full_size(ops):
     assert rv <= 1e-5 and rx <= 1e-5
     assert ro <= 1e-11
     assert rd <= 1e-6
+
+
+def test_large_sky_quads_vs_fp64(ops):
+    """nside 2048 (50,331,648 pixels): the quad layout at scale (every pixel
+    off the equator in a quad: (P/2 - 2 nside)/2 quads) and the mixed quad
+    kernels against the all-FP64 per-pixel kernels on 2,100 rows, relative
+    L2 <= 1e-5."""
+    nside = 2048
+    dev = torch.device("cuda:0")
+    th, ph = ops.pix2ang(nside, nest=False, with_ring=False)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    B = torch.rand(th.numel(), dtype=torch.float64, device=dev, generator=g)
+    lmn, I = ops.pixel_dirs(th, ph, B)
+    del th, ph, B
+    sky = ops.Sky(lmn)
+    lay = sky.pairs()
+    P = sky.P
+    assert lay["uses_quads"] and lay["quads"] == (P // 2 - 2 * nside) // 2
+    rng = np.random.default_rng(1)
+    pos = geo.satellite_positions(6, 140, 30e3, True, 100, rng)
+    uvw, xa, xb, _ = geo.baseline_rows(pos, geo.C_LIGHT / 10e6)
+    c = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    t = dict(uvw=c(uvw), xa=c(xa), xb=c(xb))
+    V32 = sky.forward(I, t["uvw"], t["xa"], t["xb"], mask=True, R_moon=R)
+    V64 = sky.forward(I, t["uvw"], t["xa"], t["xb"], mask=True, R_moon=R, fp64=True)
+    x32 = sky.backproject(t["uvw"], V64, None, t["xa"], t["xb"], mask=True, R_moon=R)
+    x64 = sky.backproject(t["uvw"], V64, None, t["xa"], t["xb"], mask=True, R_moon=R, fp64=True)
+    rv = float(torch.linalg.vector_norm(V32 - V64) / torch.linalg.vector_norm(V64))
+    rx = float(torch.linalg.vector_norm(x32 - x64) / torch.linalg.vector_norm(x64))
+    print(f"nside {nside}: {lay}; forward {rv:.2e}, back-projection {rx:.2e}", flush=True)
+    assert rv <= 1e-5 and rx <= 1e-5
