@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_group_tiles(const float4* __restrict__ 
       keep = d + c.w + gc.w >= cut - kClsEps;
       if (both_sides) keep |= -d + c.w + gc.w >= cut - kClsEps;
     }
-    const uint64_t bal = __ballot(keep);
+    const uint64_t bal = wave_ballot(keep);
     if (lane == 0) wcount[wv] = __popcll(bal);
     __syncthreads();
     int before = running;
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_group_tiles2(const float4* __restrict__
         keep |= (d + c.w + gc.w >= cut - kClsEps) | (-d + c.w + gc.w >= cut - kClsEps);
       }
     }
-    const uint64_t bal = __ballot(keep);
+    const uint64_t bal = wave_ballot(keep);
     if (lane == 0) wcount[wv] = __popcll(bal);
     __syncthreads();
     int before = running;

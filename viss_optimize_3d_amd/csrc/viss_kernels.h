@@ -6,6 +6,11 @@
 
 namespace viss {
 
+// Wavefront ballot of a lane predicate.  The builtin takes the predicate as a
+// bool, so a comparison result that already is a lane mask is used as it is
+// (HIP's __ballot(int) re-materialises it with a select and a compare).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // ------------------------------------------------------------ pair math
 __device__ __forceinline__ float rev_fp32(double ph) {
   return (float)__builtin_amdgcn_fract(ph);  // [0,1) revolutions, exact in FP64
@@ -100,8 +105,8 @@ __device__ __forceinline__ void forward_tiles(const SkyView& sv, const int32_t* 
       classify(cone[t], r.cl, any_v, all_v);
       any_v = any_v & r.act;
       all_v = all_v | !r.act;
-      if (__ballot(any_v) == 0) continue;
-      full = __ballot(!all_v) == 0;
+      if (wave_ballot(any_v) == 0) continue;
+      full = wave_ballot(!all_v) == 0;
     }
     const PixPack* __restrict__ tp = pix + t * kTile;
     if (!F64) {
@@ -216,15 +221,43 @@ __device__ __forceinline__ void pair_tile_forward(const PairPack* __restrict__ t
 
 // Tile classification of both sides: the a pixels lie in the cone cn, the b
 // pixels in (-centre, same chord).  Inactive lanes count as seeing nothing and
-// everything (they never force a test).
+// everything (they never force a test).  With d = centre.r_hat, a pixel s of
+// the cone has s.r_hat within d -+ w, so side a is
+//   seen somewhere   when d + w >= c - eps,   i.e. d - c >= -W,
+//   seen everywhere  when d - w >= c + eps,   i.e. d - c >=  W,
+// and side b (-d -+ w) seen somewhere when d + c <= W, everywhere when
+// d + c <= -W, with W = w + eps.  d - c comes out of the dot product's FMA
+// chain; d + c = (d - c) + 2c, one more rounding, far inside eps.
 __device__ __forceinline__ void classify_pair(const float4 cn, const VisCls& c, bool act, bool& anyA, bool& allA,
                                               bool& anyB, bool& allB) {
-  const float da = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, cn.z * c.az));
-  const float db = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, cn.z * c.bz));
-  anyA = act & (da + cn.w >= c.ca - kClsEps) & (db + cn.w >= c.cb - kClsEps);
-  allA = !act | ((da - cn.w >= c.ca + kClsEps) & (db - cn.w >= c.cb + kClsEps));
-  anyB = act & (-da + cn.w >= c.ca - kClsEps) & (-db + cn.w >= c.cb - kClsEps);
-  allB = !act | ((-da - cn.w >= c.ca + kClsEps) & (-db - cn.w >= c.cb + kClsEps));
+  const float W = cn.w + kClsEps;
+  const float ea = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, fmaf(cn.z, c.az, -c.ca)));
+  const float eb = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, fmaf(cn.z, c.bz, -c.cb)));
+  const float sa = ea + 2.f * c.ca, sb = eb + 2.f * c.cb;
+  anyA = act & (ea >= -W) & (eb >= -W);
+  allA = !act | ((ea >= W) & (eb >= W));
+  anyB = act & (sa <= W) & (sb <= W);
+  allB = !act | ((sa <= -W) & (sb <= -W));
+}
+
+// classify_pair reduced over the wavefront (rows in lanes; actm = the lanes
+// holding a row): side a / b seen by some row somewhere, by every row
+// everywhere.  Each comparison is balloted on its own, so it stays the lane
+// mask its compare instruction writes, and the masks combine in scalar code.
+struct PairSides {
+  bool seeA, wholeA, seeB, wholeB;
+};
+
+__device__ __forceinline__ PairSides classify_pair_wave(const float4 cn, const VisCls& c, uint64_t actm) {
+  const float W = cn.w + kClsEps;
+  const float ea = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, fmaf(cn.z, c.az, -c.ca)));
+  const float eb = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, fmaf(cn.z, c.bz, -c.cb)));
+  const float sa = ea + 2.f * c.ca, sb = eb + 2.f * c.cb;
+  const uint64_t anyA = wave_ballot(ea >= -W) & wave_ballot(eb >= -W) & actm;
+  const uint64_t allA = wave_ballot(ea >= W) & wave_ballot(eb >= W);
+  const uint64_t anyB = wave_ballot(sa <= W) & wave_ballot(sb <= W) & actm;
+  const uint64_t allB = wave_ballot(sa <= -W) & wave_ballot(sb <= -W);
+  return PairSides{anyA != 0, (actm & ~allA) == 0, anyB != 0, (actm & ~allB) == 0};
 }
 
 // Accumulates the pair tiles t = list[j] (t = j when list is NULL), j in
@@ -234,16 +267,15 @@ template <bool MASK>
 __device__ __forceinline__ void forward_pair_tiles(const PairPack* __restrict__ pp, const float4* __restrict__ pcone,
                                                    const int32_t* __restrict__ list, int64_t j0, int64_t j1,
                                                    const FwdRow& r, double& Sr, double& Si) {
+  const uint64_t actm = wave_ballot(r.act);
   for (int64_t jt = j0; jt < j1; ++jt) {
     const int64_t t = list ? (int64_t)list[jt] : jt;
     int ma = kSideFull, mb = kSideFull;
     if (MASK) {
-      bool anyA, allA, anyB, allB;
-      classify_pair(pcone[t], r.cl, r.act, anyA, allA, anyB, allB);
-      const bool seeA = __ballot(anyA) != 0, seeB = __ballot(anyB) != 0;
-      if (!seeA && !seeB) continue;
-      ma = !seeA ? kSideNone : (__ballot(!allA) == 0 ? kSideFull : kSideTest);
-      mb = !seeB ? kSideNone : (__ballot(!allB) == 0 ? kSideFull : kSideTest);
+      const PairSides ps = classify_pair_wave(pcone[t], r.cl, actm);
+      if (!ps.seeA && !ps.seeB) continue;
+      ma = !ps.seeA ? kSideNone : (ps.wholeA ? kSideFull : kSideTest);
+      mb = !ps.seeB ? kSideNone : (ps.wholeB ? kSideFull : kSideTest);
     }
     const PairPack* __restrict__ tp = pp + t * 64;
     float ar = 0.f, ai = 0.f;
@@ -269,9 +301,13 @@ __device__ __forceinline__ void forward_pair_tiles(const PairPack* __restrict__ 
 // -A - C.  Per ring, R = sum (wa + wb) cos A and J = sum (wa - wb) sin A, and
 // the tile adds e^{-i C}(Rz - i Jz) + e^{+i C}(Rm - i Jm).
 // Mode of one ring for every lane of the wavefront: nothing seen, both
-// pixels seen whole, only the first (a or c), only the second (b or d), or
-// the exact per-pixel test.
-enum RingMode { kRingNone = 0, kRingFF = 1, kRingFN = 2, kRingNF = 3, kRingTest = 4 };
+// pixels seen whole, only the first (a or c), only the second (b or d), the
+// first tested per pixel and the second seen whole (TF), the reverse (FT), or
+// both tested.  The occultation cap is larger than a hemisphere (its cut is
+// negative), so the antipode of a pixel near the cap's edge is seen: a ring
+// that needs a test almost always has one side tested and the other whole.
+enum RingMode { kRingNone = 0, kRingFF = 1, kRingFN = 2, kRingNF = 3, kRingTF = 4, kRingFT = 5, kRingTest = 6 };
+constexpr int kRingModes = 7;
 
 // Per-lane thresholds of a tile: pixel (x, y, z) passes satellite a when
 // x xa + y ya >= ta - z za = t1a, (-x, -y, z) when x xa + y ya <= -t1a,
@@ -292,6 +328,14 @@ __device__ __forceinline__ void ring_step(float w1, float w2, float sum, float d
   } else if (MR == kRingNF) {
     R = fmaf(w2, cs, R);
     J = fmaf(-w2, sn, J);
+  } else if (MR == kRingTF) {
+    const float a = v1 ? w1 : 0.f;
+    R = fmaf(a + w2, cs, R);
+    J = fmaf(a - w2, sn, J);
+  } else if (MR == kRingFT) {
+    const float b = v2 ? w2 : 0.f;
+    R = fmaf(w1 + b, cs, R);
+    J = fmaf(w1 - b, sn, J);
   } else if (MR == kRingTest) {
     const float a = v1 ? w1 : 0.f, b = v2 ? w2 : 0.f;
     R = fmaf(a + b, cs, R);
@@ -309,7 +353,7 @@ __device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ t
     float sn, cs;
     sincos_rev(rev_fp32(fma(u, q.x, v * q.y)), sn, cs);
     double pa = 0.0, pb = 0.0;
-    if (MZ == kRingTest || MM == kRingTest) {
+    if (MZ >= kRingTF || MM >= kRingTF) {
       pa = fma(q.x, mk.xa, q.y * mk.ya);
       pb = fma(q.x, mk.xb, q.y * mk.yb);
     }
@@ -322,33 +366,34 @@ __device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ t
 
 __device__ __forceinline__ int ring_mode(bool see1, bool whole1, bool see2, bool whole2) {
   const bool t1 = see1 & !whole1, t2 = see2 & !whole2;
-  return (!see1 & !see2) ? kRingNone
-         : (t1 | t2)     ? kRingTest
-         : (see1 & see2) ? kRingFF
-         : see1          ? kRingFN
-                         : kRingNF;
+  return (!see1 & !see2)      ? kRingNone
+         : (t1 & !t2 & whole2) ? kRingTF
+         : (t2 & !t1 & whole1) ? kRingFT
+         : (t1 | t2)           ? kRingTest
+         : (see1 & see2)       ? kRingFF
+         : see1                ? kRingFN
+                               : kRingNF;
 }
 
 // Accumulates the quad tiles [t0, t1) into (Sr, Si) (the convention of
-// forward_store, without the phase-tracking term).
+// forward_store, without the phase-tracking term).  The ring-factor products
+// of up to 8 tiles are summed in FP32 before they join the FP64 sums.
 template <bool MASK>
 __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
                                                    const float4* __restrict__ qcone, int64_t t0, int64_t t1,
                                                    const FwdRow& r, double& Sr, double& Si) {
+  float fr = 0.f, fi = 0.f;
+  int nacc = 0;
+  const uint64_t actm = wave_ballot(r.act);
   for (int64_t t = t0; t < t1; ++t) {
     const double z = qz[t];
     int mz = kRingFF, mm = kRingFF;
     QuadThr th{0.0, 0.0, 0.0, 0.0};
     if (MASK) {
-      bool anyA, allA, anyD, allD, anyB, allB, anyC, allC;
-      classify_pair(qcone[2 * t], r.cl, r.act, anyA, allA, anyD, allD);      // a and its antipode d
-      classify_pair(qcone[2 * t + 1], r.cl, r.act, anyB, allB, anyC, allC);  // b and its antipode c
-      const bool sA = __ballot(anyA) != 0, sB = __ballot(anyB) != 0, sC = __ballot(anyC) != 0,
-                 sD = __ballot(anyD) != 0;
-      const bool wA = __ballot(!allA) == 0, wB = __ballot(!allB) == 0, wC = __ballot(!allC) == 0,
-                 wD = __ballot(!allD) == 0;
-      mz = ring_mode(sA, wA, sB, wB);
-      mm = ring_mode(sC, wC, sD, wD);
+      const PairSides ad = classify_pair_wave(qcone[2 * t], r.cl, actm);      // a and its antipode d
+      const PairSides bc = classify_pair_wave(qcone[2 * t + 1], r.cl, actm);  // b and its antipode c
+      mz = ring_mode(ad.seeA, ad.wholeA, bc.seeA, bc.wholeA);
+      mm = ring_mode(bc.seeB, bc.wholeB, ad.seeB, ad.wholeB);
       if (mz == kRingNone && mm == kRingNone) continue;
       const double za = z * r.mk.za, zb = z * r.mk.zb;
       th.t1a = r.mk.ta - za;
@@ -359,24 +404,33 @@ __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ 
     const QuadPack* __restrict__ tq = qp + t * kQuadTile;
     float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
 #define VISS_QT(A, B)                                                                      \
-  case A * 5 + B:                                                                          \
+  case A * kRingModes + B:                                                                 \
     quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, th, Rz, Jz, Rm, Jm);                      \
     break;
 #define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
-                       VISS_QT(A, kRingTest)
-    switch (mz * 5 + mm) {  // wave-uniform
+                       VISS_QT(A, kRingTF) VISS_QT(A, kRingFT) VISS_QT(A, kRingTest)
+    switch (mz * kRingModes + mm) {  // wave-uniform
       VISS_QT(kRingNone, kRingFF) VISS_QT(kRingNone, kRingFN) VISS_QT(kRingNone, kRingNF)
-      VISS_QT(kRingNone, kRingTest)
-      VISS_QT_ROW(kRingFF) VISS_QT_ROW(kRingFN) VISS_QT_ROW(kRingNF) VISS_QT_ROW(kRingTest)
+      VISS_QT(kRingNone, kRingTF) VISS_QT(kRingNone, kRingFT) VISS_QT(kRingNone, kRingTest)
+      VISS_QT_ROW(kRingFF) VISS_QT_ROW(kRingFN) VISS_QT_ROW(kRingNF) VISS_QT_ROW(kRingTF) VISS_QT_ROW(kRingFT)
+      VISS_QT_ROW(kRingTest)
       default: break;
     }
 #undef VISS_QT_ROW
 #undef VISS_QT
     float sc, cc;
     sincos_rev(rev_fp32(r.w * z), sc, cc);
-    Sr += (double)(cc * (Rz + Rm)) + (double)(sc * (Jm - Jz));
-    Si += (double)(sc * (Rz - Rm)) + (double)(cc * (Jz + Jm));
+    fr = fmaf(cc, Rz + Rm, fmaf(sc, Jm - Jz, fr));
+    fi = fmaf(sc, Rz - Rm, fmaf(cc, Jz + Jm, fi));
+    if (++nacc == 8) {
+      Sr += (double)fr;
+      Si += (double)fi;
+      fr = fi = 0.f;
+      nacc = 0;
+    }
   }
+  Sr += (double)fr;
+  Si += (double)fi;
 }
 
 // Applies the phase-tracking term, V e^{-2 pi i c0} = (Sr - i Si)(cos - i sin),

@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
     if (MASK) {
       bool any_v = false, all_v = false;
       if (lane < nk) classify(cn, vc[kb + lane], any_v, all_v);
-      anym = __ballot(any_v);
-      fullm = __ballot(all_v);
+      anym = wave_ballot(any_v);
+      fullm = wave_ballot(all_v);
     } else {
       anym = nk == 64 ? ~0ull : ((1ull << nk) - 1ull);
       fullm = anym;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
     if (MASK) {
       bool anyA = false, allA = false, anyB = false, allB = false;
       if (lane < nk) classify_pair(cn, vc[kb + lane], true, anyA, allA, anyB, allB);
-      const uint64_t sA = __ballot(anyA), fA = __ballot(allA), sB = __ballot(anyB), fB = __ballot(allB);
+      const uint64_t sA = wave_ballot(anyA), fA = wave_ballot(allA), sB = wave_ballot(anyB), fB = wave_ballot(allB);
       both = fA & fB;
       onlyA = fA & ~sB;
       onlyB = fB & ~sA;
@@ -362,6 +362,14 @@ __device__ __forceinline__ void ring_bp(float yr, float yi, float cs, float sn, 
     S1 = fmaf(yr, cs, fmaf(-yi, sn, S1));
   } else if (MR == kRingNF) {
     S2 = fmaf(yr, cs, fmaf(yi, sn, S2));
+  } else if (MR == kRingTF) {
+    const float t1 = fmaf(yr, cs, -yi * sn);
+    S1 += v1 ? t1 : 0.f;
+    S2 = fmaf(yr, cs, fmaf(yi, sn, S2));
+  } else if (MR == kRingFT) {
+    const float t2 = fmaf(yr, cs, yi * sn);
+    S1 = fmaf(yr, cs, fmaf(-yi, sn, S1));
+    S2 += v2 ? t2 : 0.f;
   } else if (MR == kRingTest) {
     const float t1 = fmaf(yr, cs, -yi * sn), t2 = fmaf(yr, cs, yi * sn);
     S1 += v1 ? t1 : 0.f;
@@ -381,7 +389,7 @@ __device__ __forceinline__ void quad_bp_rows(uint64_t msk, const VisHot* __restr
     float sn, cs;
     sincos_rev(rev_fp32(fma(h.u, qx, h.v * qy)), sn, cs);
     bool va = false, vb = false, vc = false, vd = false;
-    if (MZ == kRingTest || MM == kRingTest) {
+    if (MZ >= kRingTF || MM >= kRingTF) {  // unused tests are dropped by the compiler
       const VisMask mk = vm[kb + j];
       const double pa = fma(qx, mk.xa, qy * mk.ya), pb = fma(qx, mk.xb, qy * mk.yb);
       const double za = zl * mk.za, zb = zl * mk.zb;
@@ -437,7 +445,7 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
   float4* __restrict__ rot_w = &rot_s[wv][0][0];
   for (int64_t kb = k0; kb < k1; kb += 64) {
     const int nk = (int)min((int64_t)64, k1 - kb);
-    int cat = 0;
+    int mz = kRingNone, mm = kRingNone;
     if (lane < nk) {
       const VisHot h = hot[kb + lane];
       const double zg[4] = {z0, z1, z2, z3};
@@ -453,9 +461,10 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
         const VisCls cl = vc[kb + lane];
         classify_pair(cnA, cl, true, anyA, allA, anyD, allD);
         classify_pair(cnB, cl, true, anyB, allB, anyC, allC);
-        cat = ring_mode(anyA, allA, anyB, allB) * 5 + ring_mode(anyC, allC, anyD, allD);
+        mz = ring_mode(anyA, allA, anyB, allB);
+        mm = ring_mode(anyC, allC, anyD, allD);
       } else {
-        cat = kRingFF * 5 + kRingFF;
+        mz = mm = kRingFF;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -463,19 +472,31 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float4* __restrict__ rot = rot_w + g * 64;
     QuadBpAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#define VISS_QB(A, B) quad_bp_rows<A, B>(__ballot(cat == A * 5 + B), hot, vm, rot, kb, qx, qy, zl, acc);
-    VISS_QB(kRingFF, kRingFF)
-    if (MASK) {
-      VISS_QB(kRingNone, kRingFF) VISS_QB(kRingNone, kRingFN) VISS_QB(kRingNone, kRingNF)
-      VISS_QB(kRingNone, kRingTest)
-      VISS_QB(kRingFF, kRingNone) VISS_QB(kRingFF, kRingFN) VISS_QB(kRingFF, kRingNF) VISS_QB(kRingFF, kRingTest)
-      VISS_QB(kRingFN, kRingNone) VISS_QB(kRingFN, kRingFF) VISS_QB(kRingFN, kRingFN) VISS_QB(kRingFN, kRingNF)
-      VISS_QB(kRingFN, kRingTest)
-      VISS_QB(kRingNF, kRingNone) VISS_QB(kRingNF, kRingFF) VISS_QB(kRingNF, kRingFN) VISS_QB(kRingNF, kRingNF)
-      VISS_QB(kRingNF, kRingTest)
-      VISS_QB(kRingTest, kRingNone) VISS_QB(kRingTest, kRingFF) VISS_QB(kRingTest, kRingFN)
-      VISS_QB(kRingTest, kRingNF) VISS_QB(kRingTest, kRingTest)
+    // rows by (ring z mode, ring -z mode): 7 + 7 ballots, and each category's
+    // rows are the AND of two of them; a ring z mode no row has skips its 7
+    uint64_t bz[kRingModes], bm[kRingModes];
+#pragma unroll
+    for (int i = 0; i < kRingModes; ++i) {
+      bz[i] = wave_ballot(mz == i);
+      bm[i] = wave_ballot(mm == i);
     }
+#define VISS_QB(A, B) quad_bp_rows<A, B>(bz[A] & bm[B], hot, vm, rot, kb, qx, qy, zl, acc);
+#define VISS_QB_ROW(A)                                                                                   \
+  if (bz[A]) {                                                                                           \
+    VISS_QB(A, kRingNone) VISS_QB(A, kRingFF) VISS_QB(A, kRingFN) VISS_QB(A, kRingNF) VISS_QB(A, kRingTF) \
+    VISS_QB(A, kRingFT) VISS_QB(A, kRingTest)                                                             \
+  }
+    if (MASK) {
+      if (bz[kRingNone]) {
+        VISS_QB(kRingNone, kRingFF) VISS_QB(kRingNone, kRingFN) VISS_QB(kRingNone, kRingNF)
+        VISS_QB(kRingNone, kRingTF) VISS_QB(kRingNone, kRingFT) VISS_QB(kRingNone, kRingTest)
+      }
+      VISS_QB_ROW(kRingFF) VISS_QB_ROW(kRingFN) VISS_QB_ROW(kRingNF) VISS_QB_ROW(kRingTF) VISS_QB_ROW(kRingFT)
+      VISS_QB_ROW(kRingTest)
+    } else {
+      VISS_QB(kRingFF, kRingFF)
+    }
+#undef VISS_QB_ROW
 #undef VISS_QB
     sa += (double)(acc.Pz - acc.Qz + acc.Az);
     sb += (double)(acc.Pz + acc.Qz + acc.Bz);

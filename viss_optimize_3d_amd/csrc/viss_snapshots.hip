@@ -225,8 +225,8 @@ __global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restr
     if (MASK) {
       bool any_v = false, all_v = false;
       if (lane < ns) classify_snapshot<NMAX>(cn, cls + (sb + lane) * NMAX, N, any_v, all_v);
-      anym = __ballot(any_v);
-      fullm = __ballot(all_v);
+      anym = wave_ballot(any_v);
+      fullm = wave_ballot(all_v);
     } else {
       anym = ns == 64 ? ~0ull : ((1ull << ns) - 1ull);
       fullm = anym;
@@ -358,8 +358,8 @@ __global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap(
     if (MASK) {
       bool any_v = false, all_v = false;
       if (lane < nt) classify_snapshot<NMAX>(cone[tb + lane], cls + s * NMAX, N, any_v, all_v);
-      anym = __ballot(any_v);
-      fullm = __ballot(all_v);
+      anym = wave_ballot(any_v);
+      fullm = wave_ballot(all_v);
     } else {
       anym = nt == 64 ? ~0ull : ((1ull << nt) - 1ull);
       fullm = anym;
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_backproject_snap_pairs(const PairPack* 
     if (MASK) {
       bool anyA = false, allA = false, anyB = false, allB = false;
       if (lane < ns) classify_snapshot_pair<NMAX>(cn, cls + (sb + lane) * NMAX, N, anyA, allA, anyB, allB);
-      const uint64_t sA = __ballot(anyA), fA = __ballot(allA), sB = __ballot(anyB), fB = __ballot(allB);
+      const uint64_t sA = wave_ballot(anyA), fA = wave_ballot(allA), sB = wave_ballot(anyB), fB = wave_ballot(allB);
       both = fA & fB;
       onlyA = fA & ~sB;
       onlyB = fB & ~sA;
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256, NMAX <= 10 ? 3 : 1) void k_forward_snap_pairs(
     if (MASK) {
       bool anyA = false, allA = false, anyB = false, allB = false;
       if (lane < nt) classify_snapshot_pair<NMAX>(pcone[tb + lane], cls + s * NMAX, N, anyA, allA, anyB, allB);
-      const uint64_t sA = __ballot(anyA), fA = __ballot(allA), sB = __ballot(anyB), fB = __ballot(allB);
+      const uint64_t sA = wave_ballot(anyA), fA = wave_ballot(allA), sB = wave_ballot(anyB), fB = wave_ballot(allB);
       both = fA & fB;
       onlyA = fA & ~sB;
       onlyB = fB & ~sA;
