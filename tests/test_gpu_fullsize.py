@@ -10,7 +10,9 @@ need the oracle to finish the whole problem:
     snapshots, 10 MHz, sigma 50 km): the mixed kernels against the all-FP64
     kernels over every row and pixel (7.1e11 pair evaluations each way), the
     FP64 forward against the oracle on 256 rows, and the adjoint identity
-    <A x, y> = <x, A^H y> of the mixed pair at full size;
+    <A x, y> = <x, A^H y> of the mixed pair at full size; and the occultation
+    decisions of the mixed quad kernels, exactly, through visible-pixel counts
+    (zero baselines, unit inputs) against the all-FP64 kernels;
   * a 50M-pixel sky (nside 2048): the quad layout at scale, mixed vs FP64.
 
 Tolerances: mixed vs FP64 (GPU or oracle) relative L2 <= 1e-5 (the precision
@@ -149,3 +151,29 @@ def test_large_sky_quads_vs_fp64(ops):
     rx = float(torch.linalg.vector_norm(x32 - x64) / torch.linalg.vector_norm(x64))
     print(f"nside {nside}: {lay}; forward {rv:.2e}, back-projection {rx:.2e}", flush=True)
     assert rv <= 1e-5 and rx <= 1e-5
+
+
+def test_precision_mask_counts_full_size(ops):
+    """Occultation decisions at full size, exactly: with zero baselines and
+    unit brightness / visibilities the forward returns each row's visible-pixel
+    count and the back-projection each pixel's seeing-row count (every phase
+    is 0).  The mixed quad kernels must give the all-FP64 per-pixel kernels'
+    counts exactly over all 900,000 rows and 786,432 pixels (the FP64 kernels'
+    decisions are pinned against the oracle in test_gpu_mask_exact.py)."""
+    wl, inp, lmn, I, sky, (uvw, xa, xb) = _setup(ops, "precision-nside256")
+    K, P = uvw.shape[1], sky.P
+    assert sky.pairs()["uses_quads"]
+    zero = torch.zeros_like(uvw)
+    ones_p = torch.ones(P, dtype=torch.float64, device="cuda")
+    ones_k = torch.ones(K, dtype=torch.complex128, device="cuda")
+    V32 = sky.forward(ones_p, zero, xa, xb, mask=True, R_moon=R)
+    V64 = sky.forward(ones_p, zero, xa, xb, mask=True, R_moon=R, fp64=True)
+    x32 = sky.backproject(zero, ones_k, None, xa, xb, mask=True, R_moon=R)
+    x64 = sky.backproject(zero, ones_k, None, xa, xb, mask=True, R_moon=R, fp64=True)
+    nrow = int(torch.count_nonzero(V32.real != V64.real))
+    npix = int(torch.count_nonzero(x32 != x64))
+    total = float(V64.real.sum())
+    print(f"  visible pairs {total:.6e}; rows with a different count {nrow}, pixels {npix}", flush=True)
+    assert total == float(x64.sum()) and total == round(total)
+    assert nrow == 0 and npix == 0
+    assert bool(torch.all(V32.imag == 0))
