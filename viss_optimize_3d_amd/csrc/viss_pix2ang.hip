@@ -29,8 +29,10 @@ __device__ inline uint32_t compact_even_bits(uint32_t v) {
   return v;
 }
 
+// floor(sqrt(x)) for 0 <= x < 2^31: the FP32 root (x rounded to 24 bits, a
+// root within an ulp) is within 1 of the answer, and the two loops correct it.
 __device__ inline int32_t isqrt32(int32_t x) {
-  int32_t s = (int32_t)sqrt((double)x);
+  int32_t s = (int32_t)__builtin_amdgcn_sqrtf((float)x);
   while ((int64_t)s * s > x) --s;
   while ((int64_t)(s + 1) * (s + 1) <= x) ++s;
   return s;
@@ -42,13 +44,14 @@ struct RingPos {
 
 // NEST: face / in-face (ix, iy) -> ring jr and in-ring jp (pix2ang.cu:99-157,
 // 0-based tables jrll/jpll of pix2ang.cu:183-184).  nside = 2^order.
+// The tables as arithmetic: jrll = {2,2,2,2, 3,3,3,3, 4,4,4,4} = 2 + face / 4
+// and jpll = {1,3,5,7, 0,2,4,6, 1,3,5,7} = 2 (face mod 4) + 1 - (face / 4 mod 2).
 __device__ inline RingPos nest_pos(int32_t ip, int32_t nside, int32_t order) {
-  constexpr int8_t jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
-  constexpr int8_t jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
   const int32_t face = ip >> (2 * order), ipf = ip & ((1 << (2 * order)) - 1);
+  const int32_t jrll = 2 + (face >> 2), jpll = 2 * (face & 3) + 1 - ((face >> 2) & 1);
   const int32_t ix = (int32_t)compact_even_bits((uint32_t)ipf), iy = (int32_t)compact_even_bits((uint32_t)ipf >> 1);
   RingPos r;
-  r.jr = jrll[face] * nside - (ix + iy) - 1;
+  r.jr = jrll * nside - (ix + iy) - 1;
   if (r.jr < nside) {
     r.nr = r.jr;
     r.kshift = 0;
@@ -59,15 +62,17 @@ __device__ inline RingPos nest_pos(int32_t ip, int32_t nside, int32_t order) {
     r.nr = 4 * nside - r.jr;
     r.kshift = 0;
   }
-  int32_t jp = (jpll[face] * r.nr + (ix - iy) + 1 + r.kshift) / 2;
+  int32_t jp = (jpll * r.nr + (ix - iy) + 1 + r.kshift) / 2;
   if (jp > 4 * nside) jp -= 4 * nside;
   if (jp < 1) jp += 4 * nside;
   r.jp = jp;
   return r;
 }
 
-// RING: Gorski et al. 2005 sec. 4 (polar caps / equatorial belt).
-__device__ inline RingPos ring_pos(int32_t p, int32_t nside) {
+// RING: Gorski et al. 2005 sec. 4 (polar caps / equatorial belt).  lg4 =
+// log2(4 nside) when nside is a power of two (the belt's row is then a
+// shift), else -1.
+__device__ inline RingPos ring_pos(int32_t p, int32_t nside, int32_t lg4) {
   const int32_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1), nl4 = 4 * nside;
   RingPos r;
   r.kshift = 0;
@@ -78,7 +83,7 @@ __device__ inline RingPos ring_pos(int32_t p, int32_t nside) {
     r.jp = p + 1 - 2 * i * (i - 1);
   } else if (p < npix - ncap) {
     const uint32_t q = (uint32_t)(p - ncap);
-    const uint32_t row = q / (uint32_t)nl4;
+    const uint32_t row = lg4 >= 0 ? q >> lg4 : q / (uint32_t)nl4;
     r.jr = (int32_t)row + nside;
     r.jp = (int32_t)(q - row * (uint32_t)nl4) + 1;
     r.nr = nside;
@@ -145,8 +150,12 @@ __device__ inline double ring_z(int32_t jr, int32_t nr, int32_t nside) {
   return -((three_ns2 - (double)(nr * nr)) / three_ns2);
 }
 
+// A division by a power of two is exact, so it is the same bits as the
+// scaling by 2^-k (all equatorial rings when nside is a power of two).
 __device__ inline double ring_phi(const RingPos& r) {
-  return (kPi / 2.0) * ((double)r.jp - 0.5 * (double)(r.kshift + 1)) / (double)r.nr;
+  const double a = (kPi / 2.0) * ((double)r.jp - 0.5 * (double)(r.kshift + 1));
+  if ((r.nr & (r.nr - 1)) == 0) return ldexp(a, -(int)__builtin_ctz((uint32_t)r.nr));
+  return a / (double)r.nr;
 }
 
 // theta is constant on a ring: acos(z) once per ring, written to the ring's
@@ -162,15 +171,25 @@ __global__ void k_ring_theta(int32_t nside, int32_t order, double* __restrict__ 
 
 // HOLDER: theta comes from the ring's holder pixel (k_ring_theta ran first);
 // otherwise every pixel computes acos itself (small maps, where the second
-// launch would cost more than the acos it saves).
+// launch would cost more than the acos it saves).  Block b walks the pixels
+// [b chunk, (b + 1) chunk), so a thread's pixels 256 apart mostly share a
+// ring and it reloads theta only when the ring changes.
 template <bool HOLDER>
-__global__ void k_pix2ang(int32_t nside, int32_t order, int32_t npix, double* __restrict__ theta,
-                          double* __restrict__ phi, int32_t* __restrict__ ring, int32_t* __restrict__ iphi) {
-  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
-    const RingPos r = order >= 0 ? nest_pos(p, nside, order) : ring_pos(p, nside);
+__global__ void k_pix2ang(int32_t nside, int32_t order, int32_t lg4, int32_t npix, int32_t chunk,
+                          double* __restrict__ theta, double* __restrict__ phi, int32_t* __restrict__ ring,
+                          int32_t* __restrict__ iphi) {
+  const int32_t p0 = blockIdx.x * chunk, p1 = min(npix, p0 + chunk);
+  int32_t jr_held = -1;
+  double th = 0.0;
+  for (int32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const RingPos r = order >= 0 ? nest_pos(p, nside, order) : ring_pos(p, nside, lg4);
     if (HOLDER) {
       const int32_t h = ring_holder(r.jr, nside, order);
-      if (p != h) theta[p] = theta[h];  // written by k_ring_theta, earlier on the stream
+      if (r.jr != jr_held) {
+        th = theta[h];  // written by k_ring_theta, earlier on the stream
+        jr_held = r.jr;
+      }
+      if (p != h) theta[p] = th;
     } else {
       theta[p] = acos(ring_z(r.jr, r.nr, nside));
     }
@@ -220,13 +239,23 @@ extern "C" int viss_pix2ang(int64_t nside, int scheme, double* theta, double* ph
     order = 0;
     while ((1ll << order) < nside) ++order;
   }
+  int32_t lg4 = -1;  // log2(4 nside) for a power-of-two nside
+  if ((nside & (nside - 1)) == 0) {
+    lg4 = 2;
+    while ((1ll << (lg4 - 2)) < nside) ++lg4;
+  }
   hipStream_t s = as_stream(stream);
+  const unsigned grid0 = grid_1d(npix);
+  const int64_t chunk = ((npix + grid0 - 1) / grid0 + 255) / 256 * 256;  // pixels per block, whole passes of 256
+  const unsigned grid = (unsigned)((npix + chunk - 1) / chunk);
   if (npix >= (int64_t)1 << 22) {  // nside >= 592: one acos per ring
     k_ring_theta<<<grid_1d(4 * nside - 1), 256, 0, s>>>((int32_t)nside, order, theta);
     VISS_LAUNCHED(k_ring_theta);
-    k_pix2ang<true><<<grid_1d(npix), 256, 0, s>>>((int32_t)nside, order, (int32_t)npix, theta, phi, ring, iphi);
+    k_pix2ang<true><<<grid, 256, 0, s>>>((int32_t)nside, order, lg4, (int32_t)npix, (int32_t)chunk, theta, phi,
+                                         ring, iphi);
   } else {
-    k_pix2ang<false><<<grid_1d(npix), 256, 0, s>>>((int32_t)nside, order, (int32_t)npix, theta, phi, ring, iphi);
+    k_pix2ang<false><<<grid, 256, 0, s>>>((int32_t)nside, order, lg4, (int32_t)npix, (int32_t)chunk, theta, phi,
+                                          ring, iphi);
   }
   VISS_LAUNCHED(k_pix2ang);
   return VISS_OK;
