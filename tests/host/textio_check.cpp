@@ -79,6 +79,12 @@ int main() {
       "0.00000000000000000000012345 9007199254740993 9007199254740992 1e22 1e23 1e-22 1e-23 "
       "3.14159265358979323846264338327950288 -0.000123457 1.5abc 7 8";
   bad |= check_parse(edge, "edge tokens");
+  // tokens past the short-mantissa fast path: signs, specials, hex, overflow,
+  // the two sides of the smallest subnormal's rounding, and "+-1" (not a number)
+  bad |= check_parse("+inf -nan +0x1p3 -0x1.8p-2 1.7976931348623159e308 2.4703282292062327e-324 "
+                     "2.4703282292062326e-324 12345678901234567.5e-3 -98765432109876543e+5 +12345678901234567 "
+                     "+-1 9",
+                     "long mantissas and specials");
   bad |= check_parse("1 2 3 x 4 5", "stop at a word");
   bad |= check_parse("", "empty");
   bad |= check_parse("   \n\t  ", "whitespace only");

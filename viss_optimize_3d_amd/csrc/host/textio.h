@@ -115,6 +115,20 @@ inline bool parse_token(const char* p, const char* end, double& out, const char*
     next = s;
     return true;
   }
+  // std::from_chars is correctly rounded too (17-digit text, whose
+  // mantissas exceed 2^53, lands here).  It takes no '+' sign and no hex, and
+  // reports out-of-range values as errors: its result is used only when it
+  // read the whole token, else strtod decides.
+  {
+    const char* b = (p < end && *p == '+') ? p + 1 : p;
+    double v;
+    const std::from_chars_result r = std::from_chars(b, end, v);
+    if (r.ec == std::errc() && r.ptr > b && (r.ptr >= end || is_space(*r.ptr)) && (b == p || *b != '-')) {
+      out = v;
+      next = r.ptr;
+      return true;
+    }
+  }
   // strtod needs a terminated string: copy the token (bounded by whitespace)
   const char* q = p;
   while (q < end && !is_space(*q)) ++q;

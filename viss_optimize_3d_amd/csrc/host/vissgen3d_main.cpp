@@ -39,6 +39,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -226,6 +227,17 @@ int main(int argc, char** argv) {
   VISSCHK(viss_pixel_dirs(d_th, d_ph, d_B, P, scale, d_lmn, d_I, nullptr));
   viss_sky* sky = nullptr;
   VISSCHK(viss_sky_create(d_lmn, P, &sky, nullptr));
+  {
+    // The symmetric layouts need the directions' symmetries to hold to 1e-12:
+    // angle files at the reference's 6 significant digits break them and run
+    // the per-pixel kernels (write them with pix2ang --precision 17 instead).
+    int64_t npair = 0, nsingle = 0, nquad = 0, nqt = 0;
+    int use_pairs = 0, use_quads = 0;
+    VISSCHK(viss_sky_pairs(sky, &npair, &nsingle, &use_pairs));
+    VISSCHK(viss_sky_quads(sky, &nquad, &nqt, &use_quads));
+    printf("sky: %lld pixels, %lld antipodal pairs, %lld mirror quads: %s kernels\n", (long long)P,
+           (long long)npair, (long long)nquad, use_quads ? "quad" : (use_pairs ? "pair" : "per-pixel"));
+  }
 
   double *d_uvw = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_uvwf = nullptr, *d_xaf = nullptr, *d_xbf = nullptr;
   double *d_w = nullptr, *d_V = nullptr, *d_dcf = nullptr, *d_wt = nullptr;
@@ -238,26 +250,57 @@ int main(int argc, char** argv) {
   std::vector<double> hC(P);
   mkdir(o.out_dir.c_str(), 0755);
 
+  // One day's input, read and parsed on a host thread while the previous day
+  // runs on the GPU and is written (the text parse is the largest host cost
+  // of a day).
+  struct DayInput {
+    std::vector<double> uvw_r, xa_r, xb_r, bll_r;
+    size_t n_uvw = 0, n_xa = 0, n_xb = 0, n_bll = 0;
+    double s_read = 0.0;
+  };
+  using clk = std::chrono::steady_clock;
+  const auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+  const auto read_day = [&o, &since](int day) {
+    const clk::time_point t0 = clk::now();
+    const std::string tag = std::to_string(day) + "day1M.txt";
+    DayInput in;
+    in.n_uvw = read_rows(o.in_dir + "uvw" + tag, 3, 3, in.uvw_r);
+    in.n_xa = read_rows(o.in_dir + "xyza" + tag, 3, 3, in.xa_r);
+    in.n_xb = read_rows(o.in_dir + "xyzb" + tag, 3, 3, in.xb_r);
+    in.n_bll = read_rows(o.in_dir + "bll" + tag, 1, 1, in.bll_r);
+    in.s_read = since(t0);
+    return in;
+  };
+  std::vector<int> todo;
   for (int day = o.first_day; day < o.first_day + o.days; ++day) {
+    const std::string out_path = o.out_dir + "C" + std::to_string(day) + "day1M.txt";
+    if (o.resume && file_exists(out_path))
+      printf("day %d: %s exists, skipped\n", day, out_path.c_str());
+    else
+      todo.push_back(day);
+  }
+  std::future<DayInput> pending;
+  if (!todo.empty()) pending = std::async(std::launch::async, read_day, todo[0]);
+
+  for (size_t i_day = 0; i_day < todo.size(); ++i_day) {
+    const int day = todo[i_day];
     const std::string tag = std::to_string(day) + "day1M.txt";
     const std::string out_path = o.out_dir + "C" + tag;
-    if (o.resume && file_exists(out_path)) {
-      printf("day %d: %s exists, skipped\n", day, out_path.c_str());
-      continue;
-    }
-    using clk = std::chrono::steady_clock;
-    const auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
-    const clk::time_point t_read = clk::now();
-    std::vector<double> uvw_r, xa_r, xb_r, bll_r;
-    const size_t n_uvw = read_rows(o.in_dir + "uvw" + tag, 3, 3, uvw_r);
-    const size_t n_xa = read_rows(o.in_dir + "xyza" + tag, 3, 3, xa_r);
-    const size_t n_xb = read_rows(o.in_dir + "xyzb" + tag, 3, 3, xb_r);
-    const size_t n_bll = read_rows(o.in_dir + "bll" + tag, 1, 1, bll_r);
+    const clk::time_point t_wait = clk::now();
+    DayInput in = pending.get();
+    const double s_wait = since(t_wait);
+    if (i_day + 1 < todo.size()) pending = std::async(std::launch::async, read_day, todo[i_day + 1]);
+    const std::vector<double>& uvw_r = in.uvw_r;
+    const std::vector<double>& xa_r = in.xa_r;
+    const std::vector<double>& xb_r = in.xb_r;
+    const std::vector<double>& bll_r = in.bll_r;
+    const size_t n_uvw = in.n_uvw, n_xa = in.n_xa, n_xb = in.n_xb, n_bll = in.n_bll;
     printf("day %d: uvw_index %zu xyz1_index %zu xyz2_index %zu bll_index %zu\n", day, n_uvw, n_xa, n_xb, n_bll);
     if (n_xa != n_uvw || n_xb != n_uvw || n_bll != n_uvw)
       fprintf(stderr, "vissgen3d: day %d: row counts differ; missing rows read as 0\n", day);
-    const double s_read = since(t_read);
+    const double s_read = in.s_read;
     const clk::time_point t_gpu = clk::now();
+    double s_prep = 0.0, s_h2d = 0.0, s_kernels = 0.0;
     const size_t rows = n_uvw;
     const size_t amount = rows / 2;  // vissgen3d_noblockage.cu:400 (integer division)
     std::fill(hC.begin(), hC.end(), 0.0);
@@ -307,6 +350,8 @@ int main(int argc, char** argv) {
           fold = same || swapped;
         }
       }
+      s_prep = since(t_gpu);
+      const clk::time_point t_h2d = clk::now();
       if (rows > cap_rows) {
         cap_rows = rows;
         if (dev_alloc(&d_uvw, 3 * rows) || dev_alloc(&d_xa, 3 * rows) || dev_alloc(&d_xb, 3 * rows) ||
@@ -329,6 +374,8 @@ int main(int argc, char** argv) {
       }
       HIPCHK(hipMemcpy(d_w, w.data(), rows * sizeof(double), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(d_bll, bll.data(), rows * sizeof(float), hipMemcpyHostToDevice));
+      s_h2d = since(t_h2d);
+      const clk::time_point t_kernels = clk::now();
       HIPCHK(hipMemset(d_V, 0, 2 * rows * sizeof(double)));
       HIPCHK(hipMemset(d_mb, 0, nr * sizeof(uint32_t)));
       HIPCHK(hipMemset(d_bad, 0, sizeof(int64_t)));
@@ -346,6 +393,8 @@ int main(int argc, char** argv) {
                                      d_C, nullptr));
       else
         VISSCHK(viss_sky_backproject(sky, d_uvw, d_xa, d_xb, d_V, d_wt, (int64_t)rows, &geom, img, d_C, nullptr));
+      HIPCHK(hipDeviceSynchronize());
+      s_kernels = since(t_kernels);
       HIPCHK(hipMemcpy(hC.data(), d_C, P * sizeof(double), hipMemcpyDeviceToHost));
       printf("day %d: forward %zu rows, imaging %s\n", day, amount, fold ? "folded Hermitian halves" : "all rows");
     }
@@ -366,8 +415,9 @@ int main(int argc, char** argv) {
       fclose(f);
     }
     if (o.timing)
-      printf("day %d: read %.3f s (4 text files), host prep + gpu %.3f s, write %.3f s\n", day, s_read, s_gpu,
-             since(t_write));
+      printf("day %d: read %.3f s (4 text files, on a host thread; waited %.3f s for it), host prep + gpu %.3f s "
+             "(host loops %.3f, copies in %.3f, kernels %.3f), write %.3f s\n",
+             day, s_read, s_wait, s_gpu, s_prep, s_h2d, s_kernels, since(t_write));
     printf("day %d: saved %s\n", day, out_path.c_str());
   }
   viss_sky_destroy(sky);
