@@ -114,3 +114,40 @@ def test_vissgen3d_missing_day_files_give_zero_image(tmp_path):
     run(["vissgen3d", "--days", "1"], cwd=str(tmp_path))
     C = np.loadtxt(tmp_path / "3dnoblockage" / "C1day1M.txt")
     assert C.shape == (192,) and np.all(C == 0)
+
+
+def test_vissgen3d_exact_healpix(tmp_path):
+    """--exact-healpix: the reference's 6-digit angle files give, through the
+    exact HEALPix angles, the image of full-precision angle files (the same
+    directions, bit for bit), with the quad kernels; a file that is not a
+    HEALPix map is refused."""
+    nside = 8
+    P = 12 * nside * nside
+    rng = np.random.default_rng(22)
+    B = geo.make_sky(nside, rng, 5)
+    pos = geo.satellite_positions(4, 40, 10e3, True, 20, rng, orbit_radius=1737.1e3 + 300e3)
+    uvw, xa, xb, bll = geo.baseline_rows(pos, 3e8 / 1e6)
+    in_dir = tmp_path / "earth_1Mhz"
+    textio.write_dataset(str(in_dir), geo.Inputs(None, B, uvw, xa, xb, bll, pos), days=1)
+    run(["pix2ang"], cwd=str(in_dir))  # NEST, 6 significant digits
+    run(["pix2ang", "--precision", "17", "--theta", "theta17.txt", "--phi", "phi17.txt"], cwd=str(in_dir))
+    common = ["vissgen3d", "--variant", "blockage", "--days", "1", "--fp64", "--precision", "17"]
+    out6 = run(common + ["--out", "six"], cwd=str(tmp_path))
+    outx = run(common + ["--exact-healpix", "--out", "exact"], cwd=str(tmp_path))
+    out17 = run(common + ["--theta", "theta17.txt", "--phi", "phi17.txt", "--out", "full"], cwd=str(tmp_path))
+    assert "per-pixel kernels" in out6 and "quad kernels" in outx and "quad kernels" in out17
+    assert "exact HEALPix angles: NEST, nside 8" in outx
+    # same directions bit for bit; the FP64 atomics may sum in another order
+    exact = np.loadtxt(tmp_path / "exact" / "C1day1M.txt")
+    assert rel_l2(exact, np.loadtxt(tmp_path / "full" / "C1day1M.txt")) < 1e-13
+    oth, oph, _, _ = oracle.pix2ang(nside, True)
+    ref = oracle_day(str(in_dir), 1, P, oth, oph, B, blockage=True)
+    assert rel_l2(exact, ref) < 1e-10
+    # one angle moved by 1e-3 rad: not this map, refused
+    th = np.array(open(in_dir / "theta_heal.txt").read().split(), np.float64)
+    th[7] += 1e-3
+    textio.write_column(str(in_dir / "theta_bad.txt"), th)
+    r = subprocess.run([os.path.join(BIN, "vissgen3d"), "--days", "1", "--exact-healpix", "--theta", "theta_bad.txt",
+                        "--out", "bad"], cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "not HEALPix RING or NEST angles" in r.stdout

@@ -29,7 +29,9 @@
 //     (--dcf-blockage-shift: vissgen3d_blockage.cu:138);
 //   * all arithmetic on inputs is FP64 (phase) / mixed (--fp64: all FP64);
 //   * --binary also writes each image as raw FP64 (C{p}day1M.bin) for fast
-//     aggregation (vissimage_sum --binary).
+//     aggregation (vissimage_sum --binary);
+//   * --exact-healpix (opt-in) replaces angle files that are a HEALPix map's
+//     angles at reduced precision by that map's exact FP64 angles.
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 #include <sys/time.h>
@@ -75,6 +77,7 @@ struct Options {
   bool per_sat_cut = false;
   double bl_max = 100e3;  // :248
   bool dcf_shift = false, extend_nr = true, fp64 = false, resume = false, binary = false, timing = false;
+  bool exact_healpix = false;
   int image_mask = -1;  // -1: per variant
   int precision = 0;
   int device = 0;
@@ -86,7 +89,7 @@ int usage() {
           "                 [--sky FILE] [--theta FILE] [--phi FILE] [--freq HZ] [--days N] [--first-day D]\n"
           "                 [--lambda-c] [--R-moon M] [--altitude M] [--per-satellite-cut] [--bl-max M]\n"
           "                 [--dcf-blockage-shift] [--no-extend-nr] [--image-mask on|off] [--fp64]\n"
-          "                 [--precision P] [--binary] [--resume] [--timing] [--device D]\n");
+          "                 [--precision P] [--binary] [--resume] [--timing] [--exact-healpix] [--device D]\n");
   return 2;
 }
 
@@ -155,6 +158,7 @@ int main(int argc, char** argv) {
     else if (a == "--resume") o.resume = true;
     else if (a == "--binary") o.binary = true;
     else if (a == "--timing") o.timing = true;
+    else if (a == "--exact-healpix") o.exact_healpix = true;
     else if (a == "--device" && (v = next())) o.device = atoi(v);
     else return usage();
   }
@@ -221,6 +225,53 @@ int main(int argc, char** argv) {
   if (dev_alloc(&d_th, P) || dev_alloc(&d_ph, P) || dev_alloc(&d_B, P) || dev_alloc(&d_lmn, 3 * P) ||
       dev_alloc(&d_I, P) || dev_alloc(&d_C, P))
     return 1;
+  if (o.exact_healpix) {
+    // Opt-in departure from the files: when they are HEALPix angles written
+    // at reduced precision (the reference's 6 significant digits move an
+    // angle by at most 5e-6 rad), use the exact FP64 angles of the same map,
+    // whose symmetries give the quad kernels.  RING and, for a power-of-two
+    // nside, NEST order are tried; a file that is not such a map is an error.
+    const int64_t nside = (int64_t)llround(std::sqrt((double)P / 12.0));
+    if (nside < 1 || 12 * nside * nside != P) {
+      fprintf(stderr, "vissgen3d: --exact-healpix: npix %lld is not 12 nside^2\n", (long long)P);
+      return 1;
+    }
+    const double tol = 2e-5;  // rad; the pixel spacing is 1.2e-4 rad even at nside 8192
+    std::vector<double> et(P), ep(P), bt, bp;
+    double best = INFINITY;
+    int best_scheme = -1;
+    for (int scheme : {VISS_SCHEME_RING, VISS_SCHEME_NEST}) {
+      if (scheme == VISS_SCHEME_NEST && (nside & (nside - 1)) != 0) continue;
+      VISSCHK(viss_pix2ang(nside, scheme, d_th, d_ph, nullptr, nullptr, nullptr));
+      HIPCHK(hipMemcpy(et.data(), d_th, P * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ep.data(), d_ph, P * sizeof(double), hipMemcpyDeviceToHost));
+      double dev = 0.0;
+      for (int64_t i = 0; i < P; ++i) {
+        const double dt = std::fabs(th[i] - et[i]);
+        double dp = std::fabs(ph[i] - ep[i]);
+        dp = std::min(dp, 2.0 * M_PI - dp);  // phi near 0 and near 2 pi are neighbours
+        const double d = std::max(dt, dp);
+        if (!(d <= dev)) dev = d == d ? d : INFINITY;  // NaN in a file: no match
+      }
+      if (dev < best) {
+        best = dev;
+        best_scheme = scheme;
+        bt = et;
+        bp = ep;
+      }
+    }
+    if (!(best <= tol)) {
+      fprintf(stderr,
+              "vissgen3d: --exact-healpix: the angle files are not HEALPix RING or NEST angles of nside %lld "
+              "to within %g rad (closest: %g rad)\n",
+              (long long)nside, tol, best);
+      return 1;
+    }
+    th.swap(bt);
+    ph.swap(bp);
+    printf("exact HEALPix angles: %s, nside %lld (the files deviate by at most %.3g rad)\n",
+           best_scheme == VISS_SCHEME_NEST ? "NEST" : "RING", (long long)nside, best);
+  }
   HIPCHK(hipMemcpy(d_th, th.data(), P * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d_ph, ph.data(), P * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d_B, B.data(), P * sizeof(double), hipMemcpyHostToDevice));
