@@ -240,24 +240,36 @@ __device__ __forceinline__ void classify_pair(const float4 cn, const VisCls& c, 
   allB = !act | ((sa <= -W) & (sb <= -W));
 }
 
-// classify_pair reduced over the wavefront (rows in lanes; actm = the lanes
-// holding a row): side a / b seen by some row somewhere, by every row
-// everywhere.  Each comparison is balloted on its own, so it stays the lane
-// mask its compare instruction writes, and the masks combine in scalar code.
+// classify_pair as lane masks (one row per lane; actm = the lanes holding a
+// row): the rows that see side a / side b somewhere (any) and everywhere
+// (all).  Each comparison is balloted on its own, so it stays the lane mask
+// its compare instruction writes, and the masks combine in scalar code.
+struct PairSideMasks {
+  uint64_t anyA, allA, anyB, allB;
+};
+
+__device__ __forceinline__ PairSideMasks classify_pair_masks(const float4 cn, const VisCls& c, uint64_t actm) {
+  const float W = cn.w + kClsEps;
+  const float ea = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, fmaf(cn.z, c.az, -c.ca)));
+  const float eb = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, fmaf(cn.z, c.bz, -c.cb)));
+  const float sa = ea + 2.f * c.ca, sb = eb + 2.f * c.cb;
+  PairSideMasks m;
+  m.anyA = wave_ballot(ea >= -W) & wave_ballot(eb >= -W) & actm;
+  m.allA = wave_ballot(ea >= W) & wave_ballot(eb >= W) & actm;
+  m.anyB = wave_ballot(sa <= W) & wave_ballot(sb <= W) & actm;
+  m.allB = wave_ballot(sa <= -W) & wave_ballot(sb <= -W) & actm;
+  return m;
+}
+
+// The same reduced over the wavefront: side a / b seen by some row
+// somewhere, by every row everywhere.
 struct PairSides {
   bool seeA, wholeA, seeB, wholeB;
 };
 
 __device__ __forceinline__ PairSides classify_pair_wave(const float4 cn, const VisCls& c, uint64_t actm) {
-  const float W = cn.w + kClsEps;
-  const float ea = fmaf(cn.x, c.ax, fmaf(cn.y, c.ay, fmaf(cn.z, c.az, -c.ca)));
-  const float eb = fmaf(cn.x, c.bx, fmaf(cn.y, c.by, fmaf(cn.z, c.bz, -c.cb)));
-  const float sa = ea + 2.f * c.ca, sb = eb + 2.f * c.cb;
-  const uint64_t anyA = wave_ballot(ea >= -W) & wave_ballot(eb >= -W) & actm;
-  const uint64_t allA = wave_ballot(ea >= W) & wave_ballot(eb >= W);
-  const uint64_t anyB = wave_ballot(sa <= W) & wave_ballot(sb <= W) & actm;
-  const uint64_t allB = wave_ballot(sa <= -W) & wave_ballot(sb <= -W);
-  return PairSides{anyA != 0, (actm & ~allA) == 0, anyB != 0, (actm & ~allB) == 0};
+  const PairSideMasks m = classify_pair_masks(cn, c, actm);
+  return PairSides{m.anyA != 0, m.allA == actm, m.anyB != 0, m.allB == actm};
 }
 
 // Accumulates the pair tiles t = list[j] (t = j when list is NULL), j in
@@ -373,6 +385,20 @@ __device__ __forceinline__ int ring_mode(bool see1, bool whole1, bool see2, bool
          : (see1 & see2)       ? kRingFF
          : see1                ? kRingFN
                                : kRingNF;
+}
+
+// ring_mode of every lane as lane masks: m[i] = the rows whose ring takes
+// mode i (any / all as from classify_pair_masks, all within any).
+__device__ __forceinline__ void ring_mode_masks(uint64_t any1, uint64_t all1, uint64_t any2, uint64_t all2,
+                                                uint64_t actm, uint64_t m[kRingModes]) {
+  const uint64_t t1 = any1 & ~all1, t2 = any2 & ~all2, untested = ~(t1 | t2);
+  m[kRingNone] = actm & ~any1 & ~any2;
+  m[kRingTF] = t1 & ~t2 & all2;
+  m[kRingFT] = t2 & ~t1 & all1;
+  m[kRingTest] = (t1 | t2) & ~m[kRingTF] & ~m[kRingFT];
+  m[kRingFF] = any1 & any2 & untested;
+  m[kRingFN] = any1 & ~any2 & untested;
+  m[kRingNF] = ~any1 & any2 & untested;
 }
 
 // Accumulates the quad tiles [t0, t1) into (Sr, Si) (the convention of

@@ -443,43 +443,47 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
   const int64_t k1 = min(K, k0 + k_per_chunk);
   double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
   float4* __restrict__ rot_w = &rot_s[wv][0][0];
+  QuadBpAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // FP32 sums of up to 4 steps (256 rows)
+  int nstep = 0;
+  const auto flush = [&]() {
+    sa += (double)(acc.Pz - acc.Qz + acc.Az);
+    sb += (double)(acc.Pz + acc.Qz + acc.Bz);
+    sc += (double)(acc.Pm - acc.Qm + acc.Cm);
+    sd += (double)(acc.Pm + acc.Qm + acc.Dm);
+    acc = QuadBpAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    nstep = 0;
+  };
   for (int64_t kb = k0; kb < k1; kb += 64) {
     const int nk = (int)min((int64_t)64, k1 - kb);
-    int mz = kRingNone, mm = kRingNone;
-    if (lane < nk) {
-      const VisHot h = hot[kb + lane];
-      const double zg[4] = {z0, z1, z2, z3};
+    const uint64_t actm = nk >= 64 ? ~0ull : ((1ull << nk) - 1ull);  // lanes holding a row
+    const int64_t kr = kb + min(lane, nk - 1);  // lanes past the last row repeat it, outside actm
+    const VisHot h = hot[kr];
+    const double zg[4] = {z0, z1, z2, z3};
 #pragma unroll
-      for (int gg = 0; gg < 4; ++gg) {
-        float s, c;
-        sincos_rev(rev_fp32(h.w * zg[gg]), s, c);
-        rot_w[gg * 64 + lane] = make_float4(fmaf(h.yr, c, -h.yi * s), fmaf(h.yr, s, h.yi * c),
-                                            fmaf(h.yr, c, h.yi * s), fmaf(h.yi, c, -h.yr * s));
-      }
-      if (MASK) {
-        bool anyA, allA, anyD, allD, anyB, allB, anyC, allC;
-        const VisCls cl = vc[kb + lane];
-        classify_pair(cnA, cl, true, anyA, allA, anyD, allD);
-        classify_pair(cnB, cl, true, anyB, allB, anyC, allC);
-        mz = ring_mode(anyA, allA, anyB, allB);
-        mm = ring_mode(anyC, allC, anyD, allD);
-      } else {
-        mz = mm = kRingFF;
-      }
+    for (int gg = 0; gg < 4; ++gg) {
+      float s, c;
+      sincos_rev(rev_fp32(h.w * zg[gg]), s, c);
+      rot_w[gg * 64 + lane] = make_float4(fmaf(h.yr, c, -h.yi * s), fmaf(h.yr, s, h.yi * c),
+                                          fmaf(h.yr, c, h.yi * s), fmaf(h.yi, c, -h.yr * s));
+    }
+    // rows by (ring z mode, ring -z mode): each category's rows are the AND of
+    // a ring z mask and a ring -z mask; a ring z mode no row has skips its 7
+    uint64_t bz[kRingModes], bm[kRingModes];
+    if (MASK) {
+      const VisCls cl = vc[kr];
+      const PairSideMasks ad = classify_pair_masks(cnA, cl, actm);  // a, and its antipode d
+      const PairSideMasks bc = classify_pair_masks(cnB, cl, actm);  // b, and its antipode c
+      ring_mode_masks(ad.anyA, ad.allA, bc.anyA, bc.allA, actm, bz);
+      ring_mode_masks(bc.anyB, bc.allB, ad.anyB, ad.allB, actm, bm);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kRingModes; ++i) bz[i] = bm[i] = 0ull;
+      bz[kRingFF] = bm[kRingFF] = actm;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float4* __restrict__ rot = rot_w + g * 64;
-    QuadBpAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    // rows by (ring z mode, ring -z mode): 7 + 7 ballots, and each category's
-    // rows are the AND of two of them; a ring z mode no row has skips its 7
-    uint64_t bz[kRingModes], bm[kRingModes];
-#pragma unroll
-    for (int i = 0; i < kRingModes; ++i) {
-      bz[i] = wave_ballot(mz == i);
-      bm[i] = wave_ballot(mm == i);
-    }
 #define VISS_QB(A, B) quad_bp_rows<A, B>(bz[A] & bm[B], hot, vm, rot, kb, qx, qy, zl, acc);
 #define VISS_QB_ROW(A)                                                                                   \
   if (bz[A]) {                                                                                           \
@@ -498,14 +502,12 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     }
 #undef VISS_QB_ROW
 #undef VISS_QB
-    sa += (double)(acc.Pz - acc.Qz + acc.Az);
-    sb += (double)(acc.Pz + acc.Qz + acc.Bz);
-    sc += (double)(acc.Pm - acc.Qm + acc.Cm);
-    sd += (double)(acc.Pm + acc.Qm + acc.Dm);
+    if (++nstep == 4) flush();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  if (nstep > 0) flush();
   if (idx.x >= 0) unsafeAtomicAdd(&x[idx.x], herm * sa);
   if (idx.y >= 0) unsafeAtomicAdd(&x[idx.y], herm * sb);
   if (idx.z >= 0) unsafeAtomicAdd(&x[idx.z], herm * sc);
