@@ -13,7 +13,9 @@ need the oracle to finish the whole problem:
     <A x, y> = <x, A^H y> of the mixed pair at full size; and the occultation
     decisions of the mixed quad kernels, exactly, through visible-pixel counts
     (zero baselines, unit inputs) against the all-FP64 kernels;
-  * a 50M-pixel sky (nside 2048): the quad layout at scale, mixed vs FP64.
+  * a 50M-pixel sky (nside 2048): the quad layout at scale, mixed vs FP64;
+  * 40,000,000 rows (the 10 MHz drivers' per-day capacity) over nside 8:
+    mixed vs FP64, FP64 vs the oracle on a sample, exact visible counts.
 
 Tolerances: mixed vs FP64 (GPU or oracle) relative L2 <= 1e-5 (the precision
 target; measured values near 3e-7 are printed); all-FP64 GPU vs the oracle
@@ -177,3 +179,50 @@ def test_precision_mask_counts_full_size(ops):
     assert total == float(x64.sum()) and total == round(total)
     assert nrow == 0 and npix == 0
     assert bool(torch.all(V32.imag == 0))
+
+
+def test_forty_million_rows(ops):
+    """40,000,000 visibility rows (the 10 MHz drivers' per-day capacity) over
+    a small HEALPix sky (nside 8): 64-bit row indexing, launch grids and
+    buffers at that size.  Mixed vs all-FP64 over every row (relative L2 <=
+    1e-5), the FP64 forward against the oracle on 2,000 rows (<= 1e-11), and
+    the visible-pixel / seeing-row counts of the mixed quad kernels equal to
+    the FP64 kernels' (zero baselines, unit inputs; the counts reach 4e7, past
+    2^24, and stay exact because FP32 partial sums span at most 256 rows)."""
+    nside, K = 8, 40_000_000
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(17)
+    pos = geo.satellite_positions(8, -(-K // 28), 50e3, True, 100, rng)
+    uvw, xa, xb, _ = geo.baseline_rows(pos, geo.C_LIGHT / 10e6, max_rows=K)
+    assert uvw.shape[1] == K
+    th, ph = ops.pix2ang(nside, nest=False, with_ring=False)
+    B = torch.from_numpy(geo.make_sky(nside, rng)).to(dev)
+    lmn, I = ops.pixel_dirs(th, ph, B)
+    sky = ops.Sky(lmn)
+    assert sky.pairs()["uses_quads"]
+    c = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    tu, ta, tb = c(uvw), c(xa), c(xb)
+    V32 = sky.forward(I, tu, ta, tb, mask=True, R_moon=R)
+    V64 = sky.forward(I, tu, ta, tb, mask=True, R_moon=R, fp64=True)
+    rv = float(torch.linalg.vector_norm(V32 - V64) / torch.linalg.vector_norm(V64))
+    sel = np.sort(rng.choice(K, 2000, replace=False))
+    l, m, n = (lmn[i].cpu().numpy() for i in range(3))
+    Vo = oracle.forward(l, m, n, I.cpu().numpy(), uvw[:, sel], xa[:, sel], xb[:, sel], flags=oracle.MASK, R_moon=R)
+    ro = rel_l2(V64[torch.from_numpy(sel).to(dev)].cpu().numpy(), Vo)
+    x32 = sky.backproject(tu, V64, None, ta, tb, mask=True, R_moon=R)
+    x64 = sky.backproject(tu, V64, None, ta, tb, mask=True, R_moon=R, fp64=True)
+    rx = float(torch.linalg.vector_norm(x32 - x64) / torch.linalg.vector_norm(x64))
+    del V32, V64, x32, x64
+    zero = torch.zeros_like(tu)
+    ones_p = torch.ones(sky.P, dtype=torch.float64, device=dev)
+    ones_k = torch.ones(K, dtype=torch.complex128, device=dev)
+    n32 = sky.forward(ones_p, zero, ta, tb, mask=True, R_moon=R).real
+    n64 = sky.forward(ones_p, zero, ta, tb, mask=True, R_moon=R, fp64=True).real
+    p32 = sky.backproject(zero, ones_k, None, ta, tb, mask=True, R_moon=R)
+    p64 = sky.backproject(zero, ones_k, None, ta, tb, mask=True, R_moon=R, fp64=True)
+    nrow, npix = int(torch.count_nonzero(n32 != n64)), int(torch.count_nonzero(p32 != p64))
+    print(f"K={K} P={sky.P}: forward mixed vs FP64 {rv:.2e}, FP64 vs oracle {ro:.2e} (2,000 rows), back-projection "
+          f"{rx:.2e}; counts differing: {nrow} rows, {npix} pixels (max seeing rows {float(p64.max()):.0f})",
+          flush=True)
+    assert rv <= 1e-5 and rx <= 1e-5 and ro <= 1e-11
+    assert nrow == 0 and npix == 0 and float(p64.max()) > 2**24
