@@ -14,7 +14,8 @@ exact count of m_kp = 1, from viss_sky_count_visible), counted once per pass.
 Printed extras: the all-FP64 GPU direct sum on the same work, relative L2 of
 the mixed result against it and against the FP64 CPU oracle on a sample, the
 FP64 CPU oracle's own throughput on that sample (OpenMP on the host cores),
-and HBM rates of the pix2ang and weights passes.
+the reference's FP32 forward formulation on the host (throughput and its error
+against FP64), and HBM rates of the pix2ang and weights passes.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload precision-nside256]
 """
@@ -370,10 +371,25 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
                          xa[:, rr].contiguous(), xb[:, rr].contiguous(), mask=True, R_moon=R).cpu().numpy()
     rel_x = float(np.linalg.norm(xg - xs) / np.linalg.norm(xs))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # the reference's own FP32 forward (acos mask at the fixed cut angle, phase
+    # against n - 1; vissgen3d_noblockage.cu:76-102) on a quarter of the rows,
+    # and its error against the FP64 oracle of the same formulation
+    r4 = rows[:max(1, n_rows // 4)]
+    u4, a4, b4 = inp.uvw[:, r4], inp.xyz_a[:, r4], inp.xyz_b[:, r4]
+    t0 = time.perf_counter()
+    V32 = oracle.forward_ref32(l, m, n, Ih, u4, a4, b4, oracle.reference_cut_angle())
+    t32 = time.perf_counter() - t0
+    V64 = oracle.forward(l, m, n, Ih, u4, a4, b4, flags=oracle.MASK | oracle.PHASE_TRACK, R_moon=1737.1e3,
+                         orbit_radius=1737.1e3 + 300e3)
+    ref32 = {"dense_pairs_per_s": r4.size * l.size / t32, "rows": int(r4.size), "seconds": t32,
+             "rel_l2_vs_fp64_same_formulation": float(np.linalg.norm(V32 - V64) / np.linalg.norm(V64))}
+    log(f"[bench] cpu reference FP32 formulation: {r4.size} rows in {t32:.2f} s, "
+        f"rel L2 vs FP64 {ref32['rel_l2_vs_fp64_same_formulation']:.2e}")
     return {"value": vis_s / (tf + ta), "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + adjoint "
                       f"(OpenMP), {tf + ta:.1f} s",
-            "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x}
+            "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x,
+            "reference_fp32_forward": ref32}
 
 
 def batched(ops, geo, dev):
