@@ -15,7 +15,8 @@ Printed extras: the all-FP64 GPU direct sum on the same work, relative L2 of
 the mixed result against it and against the FP64 CPU oracle on a sample, the
 FP64 CPU oracle's own throughput on that sample (OpenMP on the host cores),
 the reference's FP32 forward formulation on the host (throughput and its error
-against FP64), and HBM rates of the pix2ang and weights passes.
+against FP64), the notebook-style NumPy forward on the CPU-runnable case, and
+HBM rates of the pix2ang and weights passes.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload precision-nside256]
 """
@@ -385,11 +386,26 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
              "rel_l2_vs_fp64_same_formulation": float(np.linalg.norm(V32 - V64) / np.linalg.norm(V64))}
     log(f"[bench] cpu reference FP32 formulation: {r4.size} rows in {t32:.2f} s, "
         f"rel L2 vs FP64 {ref32['rel_l2_vs_fp64_same_formulation']:.2e}")
+    # the notebook-style NumPy path (one vectorised exp over the (rows x pixels) phase matrix) on the
+    # CPU-runnable case: plain-nside16, 3,000 rows x 3,072 pixels, no occultation
+    from viss_optimize_3d_amd import geometry as geo
+
+    w1 = geo.WORKLOADS["plain-nside16"]
+    in1 = geo.generate(w1)
+    th1, ph1, _, _ = oracle.pix2ang(w1.nside, False)
+    l1, m1, n1, I1 = oracle.pixel_dirs(th1, ph1, in1.B, 4.0 * np.pi / th1.size)
+    t0 = time.perf_counter()
+    Vn = np.exp(-2j * np.pi * (in1.uvw.T @ np.stack([l1, m1, n1]))) @ I1
+    tn = time.perf_counter() - t0
+    Vo1 = oracle.forward(l1, m1, n1, I1, in1.uvw)
+    numpy1 = {"pairs_per_s": in1.uvw.shape[1] * l1.size / tn, "seconds": tn, "rows": int(in1.uvw.shape[1]),
+              "pixels": int(l1.size), "rel_l2_vs_oracle": float(np.linalg.norm(Vn - Vo1) / np.linalg.norm(Vo1))}
+    log(f"[bench] cpu NumPy notebook-style forward, {w1.name}: {tn:.2f} s, {numpy1['pairs_per_s']:.3g} pairs/s")
     return {"value": vis_s / (tf + ta), "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + adjoint "
                       f"(OpenMP), {tf + ta:.1f} s",
             "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x,
-            "reference_fp32_forward": ref32}
+            "reference_fp32_forward": ref32, "numpy_notebook_forward": numpy1}
 
 
 def batched(ops, geo, dev):
