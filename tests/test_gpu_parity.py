@@ -5,8 +5,10 @@ Tolerances (stated here, per quantity):
   * pix2ang ring / in-ring indices and phi: bit-exact; theta within 1 ulp;
   * pixel directions: 4e-16 absolute;
   * weights: gs, mb, dcf bit-exact; dg, wt within 1e-14 relative or 1e-13
-    absolute (dg = 1.5 sqrt|1 - 4 sin^2 g| / |cos g| loses digits by
-    cancellation where sin g ~ 1/2, so device and host libm ulps are amplified);
+    absolute, plus dg's own rounding sensitivity (dg = 1.5 sqrt|1 - 4 sin^2 g|
+    / |cos g| loses digits by cancellation where sin g ~ 1/2: an ulp of
+    1 - 4 sin^2 g in either evaluation moves dg by up to 1.5 sqrt(4.4e-16) /
+    |cos g|);
   * all-FP64 forward / back-projection: relative L2 <= 1e-11 (FP64 phase
     rounding alone is ~1e-12 relative at |b.s| ~ 10^4 cycles);
   * mixed-precision forward / back-projection: relative L2 <= 1e-5 (the
@@ -194,14 +196,28 @@ def test_weights_parity(ops):
     bll[:5] = [0.25, 0.75, 1.25, 333.75, 2e6]
     w = rng.uniform(-1, 1, K) * bll * 0.95
     w[7] = 5 * bll[7]  # NaN dg -> weight 1/8
+    w[3], w[9], w[10] = bll[3], bll[9], -bll[10]  # |w / bll| = 1 (row 3 in shell nr: dcf 0 with the shift)
+    w[11:20] = 0.5 * bll[11:20] * (1.0 + np.array([-1e-9, -1e-12, -1e-15, 0.0, 1e-15, 1e-12, 1e-9, 0.2, -0.2]))
     for shift in (False, True):
         out = ops.weights(cuda(bll), cuda(w), nr, blockage_shift=shift)
         gs, mb, dcf, dg, wt, bad = oracle.weights(bll, w, nr, oracle.DCF_BLOCKAGE_SHIFT if shift else 0)
         np.testing.assert_array_equal(out["gs"].cpu().numpy(), gs)
         np.testing.assert_array_equal(out["mb"].cpu().numpy().astype(np.uint32), mb)
         np.testing.assert_array_equal(out["dcf"].cpu().numpy(), dcf)
-        np.testing.assert_allclose(out["dg"].cpu().numpy(), dg, rtol=1e-14, atol=1e-13, equal_nan=True)
-        np.testing.assert_allclose(out["wt"].cpu().numpy(), wt, rtol=1e-14, atol=1e-13)
+        # dg = 1.5 sqrt|1 - 4 x^2| / sqrt(1 - x^2), x = w / bll: near |x| = 1/2 the
+        # 1 - 4 x^2 of either evaluation carries ~4.4e-16 of rounding, which the
+        # square root turns into up to 1.5 / cos(gamma) * min(sqrt(e), e / sqrt|s|)
+        x = w / bll.astype(np.float64)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            e, sa = 4.4e-16, np.abs(1.0 - 4.0 * x * x)
+            cond = 1.5 / np.sqrt(np.abs((1.0 - x) * (1.0 + x))) * np.minimum(np.sqrt(e), e / np.sqrt(sa))
+        cond = np.nan_to_num(cond, nan=0.0, posinf=0.0)
+        d_gpu, w_gpu = out["dg"].cpu().numpy(), out["wt"].cpu().numpy()
+        fin = np.isfinite(dg)
+        assert np.array_equal(np.isnan(d_gpu), np.isnan(dg))
+        assert np.all(np.abs(d_gpu[fin] - dg[fin]) <= 1e-13 + 1e-14 * np.abs(dg[fin]) + cond[fin])
+        dcf_k = np.where((gs >= 0) & (gs <= nr), dcf[np.clip(gs, 0, nr)], 0.0)
+        assert np.all(np.abs(w_gpu - wt) <= 1e-13 + 1e-14 * np.abs(wt) + dcf_k * cond)
         assert int(out["n_out_of_range"].item()) == bad
 
 

@@ -7,8 +7,8 @@
 //   k_shell_hist : gs_k (FP32, bit-exact with the reference's expression) and
 //                  an LDS-privatised uint32 histogram of gs, flushed with one
 //                  global atomic per non-empty bin per block      [reads 4 B, writes 4 B per row]
-//   k_dcf        : dcf_j from the exact shell volume               [O(nr)]
-//   k_row_weights: dg_k, wt_k = fmin(dcf[gs_k] dg_k, 1/8)           [reads 16 B, writes 8-16 B per row]
+//   k_row_weights: dg_k, wt_k = fmin(dcf[gs_k] dg_k, 1/8), gs_k recomputed from bll,
+//                  and the dcf_j table from the exact shell volumes  [reads 12 B, writes 8-16 B per row]
 // All HBM-bound; the sort is dropped (its order is never used).
 #include <cstdint>
 
@@ -55,15 +55,23 @@ __global__ void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t n
     const int64_t K4 = K / 4;
     const float4* __restrict__ b4 = reinterpret_cast<const float4*>(bll);
     int4* __restrict__ g4 = reinterpret_cast<int4*>(gs);
-    for (int64_t q = t0; q < K4; q += stride) {
-      const float4 b = b4[q];
+    auto one = [&](const float4 b, int64_t q) {
       const int4 g = make_int4(shell_index(b.x), shell_index(b.y), shell_index(b.z), shell_index(b.w));
       g4[q] = g;
       count_shell(g.x, nr, hist, mb, use_lds, bad);
       count_shell(g.y, nr, hist, mb, use_lds, bad);
       count_shell(g.z, nr, hist, mb, use_lds, bad);
       count_shell(g.w, nr, hist, mb, use_lds, bad);
+    };
+    int64_t q = t0;
+    for (; q + 3 * stride < K4; q += 4 * stride) {  // four loads in flight before the atomics
+      float4 b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[i] = b4[q + i * stride];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) one(b[i], q + i * stride);
     }
+    for (; q < K4; q += stride) one(b4[q], q);
     k_tail = K4 * 4;
   }
   for (int64_t k = k_tail + t0; k < K; k += stride) {
@@ -84,33 +92,74 @@ __global__ void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t n
   }
 }
 
-__global__ void k_dcf(const uint32_t* __restrict__ mb, int32_t nr, int shift, double* __restrict__ dcf) {
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j <= nr; j += (int64_t)gridDim.x * blockDim.x) {
-    if (j == 0) {
-      dcf[0] = 1.0 / (4.0 * kPi);
-      continue;
-    }
-    const int64_t jj = shift ? j + 1 : j;
-    if (jj > nr) {
-      dcf[j] = 0.0;
-      continue;
-    }
-    const double R = (double)jj / 2.0 + 0.25, Rm = R - 0.5;
-    dcf[j] = (2.0 * kPi / 3.0) * (R * R * R - Rm * Rm * Rm) / (double)mb[jj - 1];
-  }
+// dcf_j, j in [0, nr]: 1 / (4 pi) for j = 0, else the volume of shell jj
+// (= j, or j + 1 with the blockage shift) over its count m = mb[jj - 1]; 0 past
+// shell nr.  dcf_slot is the mb entry it reads, clamped into the table so that
+// the load can be issued before it is known whether it is needed.
+__device__ inline int64_t dcf_slot(int64_t j, int32_t nr, int shift) {
+  const int64_t jj = shift ? j + 1 : j;
+  return (jj >= 1 && jj <= nr) ? jj - 1 : 0;
 }
 
+__device__ inline double dcf_value(int64_t j, uint32_t m, int32_t nr, int shift) {
+  const int64_t jj = shift ? j + 1 : j;
+  const double R = (double)jj / 2.0 + 0.25, Rm = R - 0.5;
+  const double v = (2.0 * kPi / 3.0) * (R * R * R - Rm * Rm * Rm) / (double)m;
+  return j == 0 ? 1.0 / (4.0 * kPi) : (jj > nr ? 0.0 : v);
+}
+
+// One row's dg and wt from bll (b), w and the count of its dcf entry (m).
+// gamma = asin(x), x = w / bll: sin(gamma) = x and cos(gamma)^2 = 1 - x^2 on
+// [-1, 1], so dg = 1.5 sqrt(|1 - 4 x^2| / (1 - x^2)), NaN for |x| > 1 (as
+// asin), and no asin or sincos is evaluated.  At |x| = 1 the asin route gives
+// cos(fl(pi/2)) = 6.1e-17, not 0: that value is kept, so dg and wt stay
+// finite there as in the formula.  The row's dcf entry is evaluated as the
+// table's is (dcf_value), so it is the same value.
+__device__ inline void row_weight(float b, double wk, int32_t s, uint32_t m, int32_t nr, int shift, double& dg,
+                                  double& wt) {
+  const double x = wk / (double)b;
+  const double q = (1.0 - x) * (1.0 + x), a = fabs(1.0 - 4.0 * x * x);
+  // q = 0 only at x = +-1, where |1 - 4 x^2| = 3
+  constexpr double kDgEdge = 1.5 * 1.7320508075688772 / 6.123233995736766e-17;
+  dg = q == 0.0 ? kDgEdge : 1.5 * sqrt(a / q);
+  const double d = dcf_value(s, m, nr, shift);
+  wt = (s < 0 || s > nr) ? 0.0 : fmin(d * dg, 0.125);
+}
+
+// The same launch writes the dcf table (entries j = thread index + multiples
+// of the grid size).  VEC: bll 8-B and w / wt / dg 16-B aligned; each thread
+// takes 2 rows per step.  The shell index is recomputed from bll (the same
+// expression as k_shell_hist, so the same gs) instead of read back, and both
+// rows' mb loads are issued before the arithmetic.
+template <bool VEC>
 __global__ void k_row_weights(const float* __restrict__ bll, const double* __restrict__ w, int64_t K, int32_t nr,
-                              const int32_t* __restrict__ gs, const double* __restrict__ dcf,
+                              const uint32_t* __restrict__ mb, int shift, double* __restrict__ dcf,
                               double* __restrict__ dg_out, double* __restrict__ wt) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
-    const double g = asin(w[k] / (double)bll[k]);
-    double sg, cg;
-    sincos(g, &sg, &cg);
-    const double dg = 1.5 * sqrt(fabs(1.0 - 4.0 * sg * sg)) / fabs(cg);
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = t0; j <= nr; j += stride) dcf[j] = dcf_value(j, mb[dcf_slot(j, nr, shift)], nr, shift);
+  int64_t k_tail = 0;
+  if (VEC) {
+    const int64_t K2 = K / 2;
+    for (int64_t q = t0; q < K2; q += stride) {
+      const float2 b = reinterpret_cast<const float2*>(bll)[q];
+      const double2 wk = reinterpret_cast<const double2*>(w)[q];
+      const int32_t s0 = shell_index(b.x), s1 = shell_index(b.y);
+      const uint32_t m0 = mb[dcf_slot(s0, nr, shift)], m1 = mb[dcf_slot(s1, nr, shift)];
+      double2 dg, wo;
+      row_weight(b.x, wk.x, s0, m0, nr, shift, dg.x, wo.x);
+      row_weight(b.y, wk.y, s1, m1, nr, shift, dg.y, wo.y);
+      if (dg_out) reinterpret_cast<double2*>(dg_out)[q] = dg;
+      reinterpret_cast<double2*>(wt)[q] = wo;
+    }
+    k_tail = K2 * 2;
+  }
+  for (int64_t k = k_tail + t0; k < K; k += stride) {
+    const float b = bll[k];
+    const int32_t sk = shell_index(b);
+    double dg, wo;
+    row_weight(b, w[k], sk, mb[dcf_slot(sk, nr, shift)], nr, shift, dg, wo);
     if (dg_out) dg_out[k] = dg;
-    const int32_t s = gs[k];
-    wt[k] = (s < 0 || s > nr) ? 0.0 : fmin(dcf[s] * dg, 0.125);
+    wt[k] = wo;
   }
 }
 #pragma clang fp contract(on)
@@ -154,13 +203,17 @@ extern "C" int viss_weights(const float* bll, const double* w, int64_t K, int32_
                                                                           (unsigned long long*)n_out_of_range, use_lds);
     VISS_LAUNCHED(k_shell_hist);
   }
-  k_dcf<<<(unsigned)((nr + 1 + 255) / 256 > 4096 ? 4096 : (nr + 1 + 255) / 256), 256, 0, s>>>(
-      mb, nr, (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0, dcf);
-  VISS_LAUNCHED(k_dcf);
-  if (K > 0) {
-    int64_t g = (K + 255) / 256;
+  {  // row weights and the dcf table (also for K = 0: the table of an empty histogram)
+    const int shift = (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0;
+    const bool vec = ((uintptr_t)bll % 8 == 0) && ((uintptr_t)w % 16 == 0) && ((uintptr_t)wt % 16 == 0) &&
+                     (dg == nullptr || (uintptr_t)dg % 16 == 0);
+    int64_t g = ((vec ? K / 2 : K) + 255) / 256;
     if (g > 65536) g = 65536;
-    k_row_weights<<<(unsigned)g, 256, 0, s>>>(bll, w, K, nr, gs, dcf, dg, wt);
+    if (g < 1) g = 1;
+    if (vec)
+      k_row_weights<true><<<(unsigned)g, 256, 0, s>>>(bll, w, K, nr, mb, shift, dcf, dg, wt);
+    else
+      k_row_weights<false><<<(unsigned)g, 256, 0, s>>>(bll, w, K, nr, mb, shift, dcf, dg, wt);
     VISS_LAUNCHED(k_row_weights);
   }
   return VISS_OK;
