@@ -26,7 +26,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.cases import rel_l2
+from tests.cases import check_weights, rel_l2
 from viss_optimize_3d_amd import geometry as geo
 
 torch = pytest.importorskip("torch")
@@ -63,11 +63,7 @@ def test_blockage_workload_full_size(ops):
     bll = inp.bll.astype(np.float32)
     nr = int(oracle.shell_index(bll).max())
     out = ops.weights(torch.from_numpy(bll).cuda(), uvw[2].contiguous(), nr)
-    gs, mb, dcf, dg, wt, bad = oracle.weights(bll, inp.uvw[2], nr)
-    np.testing.assert_array_equal(out["gs"].cpu().numpy(), gs)
-    np.testing.assert_array_equal(out["mb"].cpu().numpy().astype(np.uint32), mb)
-    np.testing.assert_array_equal(out["dcf"].cpu().numpy(), dcf)
-    np.testing.assert_allclose(out["wt"].cpu().numpy(), wt, rtol=1e-14, atol=1e-13)
+    gs, mb, dcf, dg, wt, bad = check_weights(out, bll, inp.uvw[2], nr)
     print(f"  weights: nr={nr}, gs / mb / dcf bit-exact over {K} rows", flush=True)
     rng = np.random.default_rng(7)
     sel = np.sort(rng.choice(K, K // 20, replace=False))
@@ -188,12 +184,13 @@ def test_forty_million_rows(ops):
     1e-5), the FP64 forward against the oracle on 2,000 rows (<= 1e-11), and
     the visible-pixel / seeing-row counts of the mixed quad kernels equal to
     the FP64 kernels' (zero baselines, unit inputs; the counts reach 4e7, past
-    2^24, and stay exact because FP32 partial sums span at most 256 rows)."""
+    2^24, and stay exact because FP32 partial sums span at most 256 rows),
+    and the density-compensation weights of every row against the oracle."""
     nside, K = 8, 40_000_000
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(17)
     pos = geo.satellite_positions(8, -(-K // 28), 50e3, True, 100, rng)
-    uvw, xa, xb, _ = geo.baseline_rows(pos, geo.C_LIGHT / 10e6, max_rows=K)
+    uvw, xa, xb, bll = geo.baseline_rows(pos, geo.C_LIGHT / 10e6, max_rows=K)
     assert uvw.shape[1] == K
     th, ph = ops.pix2ang(nside, nest=False, with_ring=False)
     B = torch.from_numpy(geo.make_sky(nside, rng)).to(dev)
@@ -226,3 +223,10 @@ def test_forty_million_rows(ops):
           flush=True)
     assert rv <= 1e-5 and rx <= 1e-5 and ro <= 1e-11
     assert nrow == 0 and npix == 0 and float(p64.max()) > 2**24
+    del zero, ones_k, n32, n64, p32, p64, ta, tb
+    # density-compensation weights over all 4e7 rows against the oracle
+    bll32 = bll.astype(np.float32)
+    nr = int(oracle.shell_index(bll32).max())
+    out = ops.weights(c(bll32), tu[2].contiguous(), nr)
+    check_weights(out, bll32, uvw[2], nr)
+    print(f"  weights over {K} rows: nr={nr}, gs / mb / dcf bit-exact, dg / wt within tolerance", flush=True)

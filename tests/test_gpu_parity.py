@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.cases import make_case, rel_l2
+from tests.cases import check_weights, make_case, rel_l2
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -199,26 +199,21 @@ def test_weights_parity(ops):
     w[3], w[9], w[10] = bll[3], bll[9], -bll[10]  # |w / bll| = 1 (row 3 in shell nr: dcf 0 with the shift)
     w[11:20] = 0.5 * bll[11:20] * (1.0 + np.array([-1e-9, -1e-12, -1e-15, 0.0, 1e-15, 1e-12, 1e-9, 0.2, -0.2]))
     for shift in (False, True):
-        out = ops.weights(cuda(bll), cuda(w), nr, blockage_shift=shift)
-        gs, mb, dcf, dg, wt, bad = oracle.weights(bll, w, nr, oracle.DCF_BLOCKAGE_SHIFT if shift else 0)
-        np.testing.assert_array_equal(out["gs"].cpu().numpy(), gs)
-        np.testing.assert_array_equal(out["mb"].cpu().numpy().astype(np.uint32), mb)
-        np.testing.assert_array_equal(out["dcf"].cpu().numpy(), dcf)
-        # dg = 1.5 sqrt|1 - 4 x^2| / sqrt(1 - x^2), x = w / bll: near |x| = 1/2 the
-        # 1 - 4 x^2 of either evaluation carries ~4.4e-16 of rounding, which the
-        # square root turns into up to 1.5 / cos(gamma) * min(sqrt(e), e / sqrt|s|)
-        x = w / bll.astype(np.float64)
-        with np.errstate(invalid="ignore", divide="ignore"):
-            e, sa = 4.4e-16, np.abs(1.0 - 4.0 * x * x)
-            cond = 1.5 / np.sqrt(np.abs((1.0 - x) * (1.0 + x))) * np.minimum(np.sqrt(e), e / np.sqrt(sa))
-        cond = np.nan_to_num(cond, nan=0.0, posinf=0.0)
-        d_gpu, w_gpu = out["dg"].cpu().numpy(), out["wt"].cpu().numpy()
-        fin = np.isfinite(dg)
-        assert np.array_equal(np.isnan(d_gpu), np.isnan(dg))
-        assert np.all(np.abs(d_gpu[fin] - dg[fin]) <= 1e-13 + 1e-14 * np.abs(dg[fin]) + cond[fin])
-        dcf_k = np.where((gs >= 0) & (gs <= nr), dcf[np.clip(gs, 0, nr)], 0.0)
-        assert np.all(np.abs(w_gpu - wt) <= 1e-13 + 1e-14 * np.abs(wt) + dcf_k * cond)
-        assert int(out["n_out_of_range"].item()) == bad
+        check_weights(ops.weights(cuda(bll), cuda(w), nr, blockage_shift=shift), bll, w, nr, shift)
+
+
+def test_weights_shell_count_past_2_24(ops):
+    """A shell holding 20,000,000 rows: its count passes 2^24, where the
+    reference's float atomicAdd counts (vissgen3d_noblockage.cu:114-123) stop
+    changing; the uint32 counts here stay exact, and so do dcf and wt."""
+    K, nr = 20_000_000, 40
+    bll = np.full(K, 10.3, np.float32)
+    bll[::1000] = 3.3  # a second shell with 20,000 rows
+    w = np.zeros(K)
+    out = ops.weights(cuda(bll), cuda(w), nr)
+    mb = out["mb"].cpu().numpy()
+    assert mb[20] == K - K // 1000 and mb[6] == K // 1000 and mb.sum() == K
+    check_weights(out, bll, w, nr)
 
 
 def test_empty_inputs(ops):
