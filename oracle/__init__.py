@@ -74,6 +74,9 @@ def lib():
         L.oracle_backproject.restype = None
         L.oracle_count_visible.argtypes = [_d, _d, _d, _i64, _d, _d, _i64, ctypes.c_double, ctypes.c_double]
         L.oracle_count_visible.restype = ctypes.c_int64
+        L.oracle_mask_flips.argtypes = [_d, _d, _d, _i64, _d, _d, _i64, ctypes.c_float, ctypes.c_double,
+                                        ctypes.c_double, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+        L.oracle_mask_flips.restype = None
         _lib = L
     return _lib
 
@@ -266,6 +269,19 @@ def count_visible(l, m, n, xyz_a, xyz_b, R_moon, orbit_radius=0.0):
     xa = _c(xyz_a, np.float64)
     xb = _c(xyz_b, np.float64)
     return int(lib().oracle_count_visible(l, m, n, l.size, xa, xb, xa.shape[1], float(R_moon), float(orbit_radius)))
+
+
+def mask_flips(l, m, n, xyz_a, xyz_b, phi_cut, R_moon, orbit_radius=0.0):
+    """(acos_only, dot_only, both): (row, pixel) pairs that the reference's FP32
+    acos mask against Phi keeps and the FP64 dot form s.r >= t drops, the
+    reverse, and pairs both keep."""
+    l, m, n = (_c(a, np.float64) for a in (l, m, n))
+    xa = _c(xyz_a, np.float64)
+    xb = _c(xyz_b, np.float64)
+    out = np.zeros(3, np.int64)
+    lib().oracle_mask_flips(l, m, n, l.size, xa, xb, xa.shape[1], float(phi_cut), float(R_moon), float(orbit_radius),
+                            out)
+    return int(out[0]), int(out[1]), int(out[2])
 
 
 def reference_cut_angle(R_moon=1737.1e3, h=300e3):

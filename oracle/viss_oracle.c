@@ -23,6 +23,8 @@
  *       closed form, fmin NaN semantics.
  *   backproject : adjoint dot-product test <Ax,y> = <x,A^H y>, point-source
  *       closed form, mpmath brute force.
+ *   mask_flips  : its dot side equals count_visible, its acos side the count
+ *       of forward_ref32 with unit brightness and zero baselines.
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  */
@@ -251,6 +253,39 @@ void oracle_forward_ref32(const float* l, const float* m, const float* n, const 
     V_re[k] = sr;
     V_im[k] = si;
   }
+}
+
+/* Occultation decisions of the reference's FP32 form (vissgen3d_noblockage.cu:
+ * 86-94: beta_i = acosf(s.r_i / |r_i|) <= Phi for both satellites, every
+ * operand FP32) against the FP64 dot form s.r_i >= t_i (oracle_mask_threshold
+ * with R_moon / orbit_radius) on every (row, pixel) pair.  out[0]: pairs only
+ * the acos form keeps; out[1]: pairs only the dot form keeps; out[2]: pairs
+ * both keep. */
+void oracle_mask_flips(const double* l, const double* m, const double* n, int64_t P, const double* xyz_a,
+                       const double* xyz_b, int64_t K, float phi_cut, double R_moon, double orbit_radius,
+                       int64_t* out) {
+  int64_t acos_only = 0, dot_only = 0, both = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : acos_only, dot_only, both)
+  for (int64_t k = 0; k < K; ++k) {
+    const double xa = xyz_a[k], ya = xyz_a[K + k], za = xyz_a[2 * K + k];
+    const double xb = xyz_b[k], yb = xyz_b[K + k], zb = xyz_b[2 * K + k];
+    const double ta = oracle_mask_threshold(xa, ya, za, R_moon, orbit_radius);
+    const double tb = oracle_mask_threshold(xb, yb, zb, R_moon, orbit_radius);
+    const float fxa = (float)xa, fya = (float)ya, fza = (float)za, fxb = (float)xb, fyb = (float)yb, fzb = (float)zb;
+    const float na = sqrtf(fxa * fxa + fya * fya + fza * fza), nb = sqrtf(fxb * fxb + fyb * fyb + fzb * fzb);
+    for (int64_t p = 0; p < P; ++p) {
+      const float fl = (float)l[p], fm = (float)m[p], fn = (float)n[p];
+      const int ref = acosf((fl * fxa + fm * fya + fn * fza) / na) <= phi_cut &&
+                      acosf((fl * fxb + fm * fyb + fn * fzb) / nb) <= phi_cut;
+      const int dot = visible(l[p], m[p], n[p], xa, ya, za, ta) && visible(l[p], m[p], n[p], xb, yb, zb, tb);
+      acos_only += ref && !dot;
+      dot_only += dot && !ref;
+      both += ref && dot;
+    }
+  }
+  out[0] = acos_only;
+  out[1] = dot_only;
+  out[2] = both;
 }
 
 /* Shell index of one baseline length, ceilAndScale (vissgen3d_noblockage.cu:

@@ -159,6 +159,34 @@ def test_forward_ref32_close_to_fp64_at_low_frequency(oracle_lib):
     assert np.linalg.norm(V32 - V64) / np.linalg.norm(V64) < 1e-4
 
 
+def test_mask_flips_acos_vs_dot(oracle_lib):
+    """The reference's FP32 acos occultation test against the FP64 dot form,
+    on 2% of the rows of the 280,000-row blockage workload (nside 64).  The
+    counter's dot side equals count_visible, its acos side the FP32 reference
+    forward's own count (unit brightness, zero baselines: each row sums 1 per
+    kept pixel).  With the same cut the two forms differ only by FP32
+    rounding at the cap's edge (full workload: 745 of 1.38e10 pairs); the
+    reference's fixed cut against each satellite's own cut differs by
+    geometry (2.2e-3 of the pairs)."""
+    wl = geo.WORKLOADS["blockage-nside64"]
+    inp = geo.generate(wl)
+    rows = np.arange(0, inp.uvw.shape[1], 50)
+    xa, xb = inp.xyz_a[:, rows], inp.xyz_b[:, rows]
+    th, ph, _, _ = oracle.pix2ang(wl.nside, False)
+    l, m, n, _ = oracle.pixel_dirs(th, ph, np.ones_like(th), 1.0)
+    R, h = geo.REF_R_MOON, geo.REF_ALTITUDE
+    phi = oracle.reference_cut_angle(R, h)
+    a, d, b = oracle.mask_flips(l, m, n, xa, xb, phi, R, R + h)
+    pairs = rows.size * l.size
+    assert b + d == oracle.count_visible(l, m, n, xa, xb, R, R + h)
+    V = oracle.forward_ref32(l, m, n, np.ones_like(l), np.zeros((3, rows.size)), xa, xb, phi)
+    assert b + a == int(V.real.sum())
+    assert 0 < a + d < 1e-6 * pairs
+    a2, d2, b2 = oracle.mask_flips(l, m, n, xa, xb, phi, R, 0.0)
+    assert b2 + a2 == b + a
+    assert 1e-4 * pairs < a2 + d2 < 1e-2 * pairs
+
+
 # ------------------------------------------------------------------ weights
 def test_weights_closed_forms(oracle_lib):
     rng = np.random.default_rng(2)
