@@ -94,7 +94,7 @@ static_assert(sizeof(VisCls) == 32, "VisCls is 32 B");
 
 // Tile classification margin (FP32 rounding of unit vectors, cuts and cone
 // data is below 1e-6; the margin keeps every "all visible" / "none visible"
-// verdict consistent with the exact FP64 per-pixel test).
+// decision consistent with the exact FP64 per-pixel test).
 constexpr float kClsEps = 1e-5f;
 
 constexpr int kTile = 64;       // forward tile (pixels)
