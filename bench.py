@@ -265,10 +265,12 @@ def main():
         rel_v = float(torch.linalg.vector_norm(V_mixed - V) / torch.linalg.vector_norm(V))
         rel_x = float(torch.linalg.vector_norm(x_mixed - x) / torch.linalg.vector_norm(x))
         max_v = float((V_mixed - V).abs().max() / V.abs().max())
+        max_x = float((x_mixed - x).abs().max() / x.abs().max())
         out["fp64_gpu"] = {"value": v64, "unit": "pairs/s", "ms_per_step": w64 * 1e3,
                            "fwd_ms": e[0].elapsed_time(e[1]), "adj_ms": e[2].elapsed_time(e[3]),
                            "speedup_mixed_over_fp64": value / v64, "rel_l2_forward": rel_v,
-                           "rel_l2_adjoint": rel_x, "max_err_forward_rel_to_max": max_v}
+                           "rel_l2_adjoint": rel_x, "max_err_forward_rel_to_max": max_v,
+                           "max_err_adjoint_rel_to_max": max_x}
         log(f"[bench] fp64 GPU: {w64 * 1e3:.1f} ms/step; mixed/fp64 {value / v64:.2f}x; rel L2 fwd {rel_v:.2e} adj {rel_x:.2e}")
     V64 = V.clone() if not args.no_fp64 else None  # the all-FP64 results, for the snapshot paths' errors
     x64 = x.clone() if not args.no_fp64 else None
@@ -371,6 +373,7 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
     xg = sky.backproject(uvw[:, rr].contiguous(), torch.from_numpy(ys).to(uvw.device), None,
                          xa[:, rr].contiguous(), xb[:, rr].contiguous(), mask=True, R_moon=R).cpu().numpy()
     rel_x = float(np.linalg.norm(xg - xs) / np.linalg.norm(xs))
+    mx_x = float(np.max(np.abs(xg - xs)) / np.max(np.abs(xs)))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     # the reference's own FP32 forward (acos mask at the fixed cut angle, phase
     # against n - 1; vissgen3d_noblockage.cu:76-102) on a quarter of the rows,
@@ -405,6 +408,7 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
             "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + adjoint "
                       f"(OpenMP), {tf + ta:.1f} s",
             "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x,
+            "max_err_adjoint_vs_oracle": mx_x,
             "reference_fp32_forward": ref32, "numpy_notebook_forward": numpy1}
 
 
