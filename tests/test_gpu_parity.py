@@ -216,6 +216,16 @@ def test_weights_shell_count_past_2_24(ops):
     check_weights(out, bll, w, nr)
 
 
+def test_weights_shell_table_beyond_lds(ops):
+    """50,000 shells: the counts no longer fit one workgroup's LDS (160 KB on
+    gfx950), so the histogram takes its global-atomic path."""
+    rng = np.random.default_rng(21)
+    K, nr = 300_000, 50_000
+    bll = rng.uniform(0.0, nr / 2, K).astype(np.float32)
+    w = rng.uniform(-1, 1, K) * bll * 0.9
+    check_weights(ops.weights(cuda(bll), cuda(w), nr), bll, w, nr)
+
+
 def test_empty_inputs(ops):
     c = make_case(nside=4, n_snap=2, n_sat=3, seed=1)
     sky = ops.Sky(cuda(c["lmn"]))
