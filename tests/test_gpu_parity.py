@@ -226,6 +226,21 @@ def test_weights_shell_table_beyond_lds(ops):
     check_weights(ops.weights(cuda(bll), cuda(w), nr), bll, w, nr)
 
 
+def test_weights_unaligned_and_ragged(ops):
+    """Row counts that are not multiples of the 4-row / 2-row vector steps
+    (the tail loops), and views one element into their buffers (the scalar
+    kernels, taken when bll / w are not 16-B / 8-B aligned)."""
+    rng = np.random.default_rng(22)
+    nr = 667
+    for K in (1, 3, 5, 100_003):
+        bll = rng.uniform(0.0, 340.0, K + 1).astype(np.float32)
+        w = rng.uniform(-1, 1, K + 1) * bll * 0.9
+        check_weights(ops.weights(cuda(bll[:K]), cuda(w[:K]), nr), bll[:K], w[:K], nr)
+        tb, tw = cuda(bll), cuda(w)
+        assert tb[1:].data_ptr() % 16 != 0 and tw[1:].data_ptr() % 16 != 0
+        check_weights(ops.weights(tb[1:], tw[1:], nr), bll[1:], w[1:], nr)
+
+
 def test_empty_inputs(ops):
     c = make_case(nside=4, n_snap=2, n_sat=3, seed=1)
     sky = ops.Sky(cuda(c["lmn"]))
