@@ -241,6 +241,37 @@ def test_weights_unaligned_and_ragged(ops):
         check_weights(ops.weights(tb[1:], tw[1:], nr), bll[1:], w[1:], nr)
 
 
+def test_one_shot_entry_points(ops):
+    """viss_forward / viss_backproject (a temporary sky per call) through the C
+    ABI: the same sums as the reusable sky's entry points, masked, with phase
+    tracking and Hermitian rows, mixed and FP64."""
+    import ctypes
+
+    from viss_optimize_3d_amd import _lib
+
+    L = _lib.load()
+    c = make_case(nside=16, n_snap=40, n_sat=6, freq=1e6, seed=11)
+    lmn, I, uvw, xa, xb = (cuda(c[k]) for k in ("lmn", "I", "uvw", "xa", "xb"))
+    P, K = lmn.shape[1], uvw.shape[1]
+    sky = ops.Sky(lmn)
+    geom = ctypes.byref(_lib.MaskGeometry(R_MOON, 0.0))
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    y = cuda(np.random.default_rng(12).standard_normal(2 * K)).view(torch.complex128)
+    for fp64 in (False, True):
+        flags = _lib.MASK | _lib.PHASE_TRACK | _lib.HERMITIAN | (_lib.PREC_FP64 if fp64 else 0)
+        V = torch.zeros(2 * K, dtype=torch.complex128, device="cuda")
+        assert L.viss_forward(p(lmn), p(I), P, p(uvw), p(xa), p(xb), K, geom, flags, p(V), stream) == 0
+        Vs = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R_MOON, phase_track=True, hermitian=True, fp64=fp64)
+        x = torch.zeros(P, dtype=torch.float64, device="cuda")
+        assert L.viss_backproject(p(lmn), P, p(uvw), p(xa), p(xb), p(y), None, K, geom, flags, p(x), stream) == 0
+        xs = sky.backproject(uvw, y, None, xa, xb, mask=True, R_moon=R_MOON, phase_track=True, hermitian=True,
+                             fp64=fp64)
+        torch.cuda.synchronize()
+        assert rel_l2(V.cpu().numpy(), Vs.cpu().numpy()) < 1e-13
+        assert rel_l2(x.cpu().numpy(), xs.cpu().numpy()) < 1e-13
+
+
 def test_empty_inputs(ops):
     c = make_case(nside=4, n_snap=2, n_sat=3, seed=1)
     sky = ops.Sky(cuda(c["lmn"]))
