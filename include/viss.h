@@ -151,6 +151,26 @@ int viss_sky_backproject(viss_sky* sky, const double* uvw, const double* xyz_a, 
                          const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
                          double* x, void* stream);
 
+/* Allocation-free viss_sky_backproject.  No counterpart in the reference (its
+ * kernels take caller-owned buffers only): this is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as
+ * viss_sky_backproject, whose row records (32 B per row, 64 B with
+ * VISS_PREC_FP64, plus 96 B with VISS_MASK) go to a sky-owned buffer that
+ * grows on demand; here they go to a static device array of
+ * VISS_NOALLOC_SCRATCH_BYTES per device, held until the process ends.  The
+ * call allocates and frees nothing and never synchronises (its first call on a
+ * device looks up the array's address, hipGetSymbolAddress).  K above what
+ * the array holds (262,142 mixed-precision rows with VISS_MASK, 209,713 with
+ * VISS_PREC_FP64 | VISS_MASK; split larger row sets over calls, the result
+ * accumulates) returns VISS_ERR_SIZE.  Every call on one device shares that
+ * device's array: calls that overlap on one device (different streams,
+ * threads or skies without ordering) race. */
+#define VISS_NOALLOC_SCRATCH_BYTES (32 << 20)
+int viss_sky_backproject_noalloc(viss_sky* sky, const double* uvw, const double* xyz_a, const double* xyz_b,
+                                 const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
+                                 int flags, double* x, void* stream);
+
 /* Number of (k, p) pairs with mask = 1 (accumulated into *count, device
  * int64): the evaluated-pair count used for throughput accounting. */
 int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,

@@ -53,3 +53,6 @@ def test_argument_validation_without_gpu():
     assert L.viss_sky_destroy(None) == 0
     assert L.viss_sky_npix(None) == -1
     assert L.viss_sky_forward(None, None, None, None, None, 1, None, 0, None, None) == -1
+    # the allocation-free back-projection validates before touching a GPU
+    assert L.viss_sky_backproject_noalloc(None, None, None, None, None, None, 5, None, 0, None, None) == -1
+    assert b"sky" in L.viss_last_error()

@@ -40,6 +40,7 @@
 //    back-projection).  The per-pixel kernels above serve every other sky.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 #include "viss_kernels.h"
 
@@ -649,23 +650,34 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   return VISS_OK;
 }
 
-static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb,
-                                const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
-                                int flags, double* x, hipStream_t s) {
+// Bytes of row records the back-projection packs for K rows (k_bp_prologue).
+static size_t bp_workspace_bytes(int64_t K, int flags) {
+  const size_t hot = (flags & VISS_PREC_FP64) ? sizeof(VisHot64) : sizeof(VisHot);
+  const size_t msk = (flags & VISS_MASK) ? sizeof(VisMask) + sizeof(VisCls) : 0;
+  return (size_t)K * (hot + msk) + 256;
+}
+
+static int check_backproject_args(const double* uvw, const double* xyza, const double* xyzb, const double* y,
+                                  int64_t K, const viss_mask_geometry* geom, int flags, double* x) {
   VISS_REQUIRE(K >= 0, "viss_backproject: negative K");
   if (K == 0) return VISS_OK;
   VISS_REQUIRE(K <= ((int64_t)1 << 40), "viss_backproject: K too large");
   VISS_REQUIRE(uvw && y && x, "viss_backproject: uvw/y/x must not be NULL");
+  VISS_REQUIRE(!(flags & VISS_MASK) || (xyza && xyzb), "viss_backproject: VISS_MASK needs xyz_a and xyz_b");
+  return validate_geom(geom, flags);
+}
+
+// The back-projection over a workspace of at least bp_workspace_bytes(K,
+// flags) bytes at `ws` (256-B aligned): the prologue packs the row records
+// there, then the tile kernels read them.  Arguments already checked.
+static int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb,
+                              const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
+                              int flags, double* x, char* ws, hipStream_t s) {
+  if (K == 0) return VISS_OK;
   const bool mask = flags & VISS_MASK;
-  VISS_REQUIRE(!mask || (xyza && xyzb), "viss_backproject: VISS_MASK needs xyz_a and xyz_b");
-  int rc = validate_geom(geom, flags);
-  if (rc != VISS_OK) return rc;
   const bool f64 = flags & VISS_PREC_FP64;
   const size_t hot_b = (size_t)K * (f64 ? sizeof(VisHot64) : sizeof(VisHot));
-  const size_t msk_b = mask ? (size_t)K * (sizeof(VisMask) + sizeof(VisCls)) : 0;
-  rc = sky->ws.reserve(hot_b + msk_b + 256, s);
-  if (rc != VISS_OK) return rc;
-  char* base = (char*)sky->ws.p;
+  char* base = ws;
   VisHot* hot = f64 ? nullptr : (VisHot*)base;
   VisHot64* hot64 = f64 ? (VisHot64*)base : nullptr;
   VisMask* vm = mask ? (VisMask*)(base + hot_b) : nullptr;
@@ -738,6 +750,45 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
   return VISS_OK;
 }
 
+// Workspace from the sky's growable buffer (allocates when K grows).
+static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb,
+                                const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
+                                int flags, double* x, hipStream_t s) {
+  int rc = check_backproject_args(uvw, xyza, xyzb, y, K, geom, flags, x);
+  if (rc != VISS_OK || K == 0) return rc;
+  rc = sky->ws.reserve(bp_workspace_bytes(K, flags), s);
+  if (rc != VISS_OK) return rc;
+  return sky_backproject_ws(sky, uvw, xyza, xyzb, y, wt, K, geom, flags, x, (char*)sky->ws.p, s);
+}
+
+// ------------------------------------------------------------ allocation-free back-projection
+// The row records of viss_sky_backproject_noalloc live in this array, one copy
+// per device for the life of the process.  Its name is a host-side stand-in:
+// host code uses only the device address hipGetSymbolAddress returns for it
+// (kept per device in g_scratch_ptr), never the name.
+static __device__ __align__(256) char scratch[VISS_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device, looked up once per device
+// (with that device current) and kept.  Allocates nothing.
+static int scratch_ptr(char** out) {
+  int dev = -1;
+  VISS_HIP(hipGetDevice(&dev));
+  VISS_REQUIRE(dev >= 0 && dev < kMaxDevices, "viss_sky_backproject_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    VISS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VISS_REQUIRE(p != nullptr, "viss_sky_backproject_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (char*)g_scratch_ptr[dev];
+  return VISS_OK;
+}
+
 }  // namespace viss
 
 using namespace viss;
@@ -759,6 +810,28 @@ int viss_sky_backproject(viss_sky* sky, const double* uvw, const double* xyz_a, 
   int rc = check_device(sky);
   if (rc != VISS_OK) return rc;
   return sky_backproject_impl(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, as_stream(stream));
+}
+
+int viss_sky_backproject_noalloc(viss_sky* sky, const double* uvw, const double* xyz_a, const double* xyz_b,
+                                 const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
+                                 int flags, double* x, void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_backproject_noalloc: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  rc = check_backproject_args(uvw, xyz_a, xyz_b, y, K, geom, flags, x);
+  if (rc != VISS_OK || K == 0) return rc;
+  const size_t need = bp_workspace_bytes(K, flags);
+  if (need > (size_t)VISS_NOALLOC_SCRATCH_BYTES) {
+    set_error("viss_sky_backproject_noalloc: %lld rows need %zu B of row records, above the %d-B static "
+              "scratch (at most %lld rows with these flags; split the rows over several calls)",
+              (long long)K, need, VISS_NOALLOC_SCRATCH_BYTES,
+              (long long)((VISS_NOALLOC_SCRATCH_BYTES - 256) / ((bp_workspace_bytes(1, flags) - 256))));
+    return VISS_ERR_SIZE;
+  }
+  char* ws = nullptr;
+  rc = scratch_ptr(&ws);
+  if (rc != VISS_OK) return rc;
+  return sky_backproject_ws(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, ws, as_stream(stream));
 }
 
 int viss_forward(const double* lmn, const double* I, int64_t P, const double* uvw, const double* xyz_a,
