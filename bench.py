@@ -41,9 +41,32 @@ PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 # Frozen flop convention per evaluated masked pair: 6 FP64 (3 FMA phase + mod-1
 # reduction) + 34 FP32 (sin + cos counted 20, complex MAC 4, two mask dots 10).
+# It charges a sin/cos to every (visibility, pixel) pair, so it is kept only as
+# the "per-pair equivalent" rate beside the roofline of the kernels that run.
 PAIR_FP64 = 6
 PAIR_FP32 = 34
 PAIR_T_PEAK = PAIR_FP32 / PEAK_FP32 + PAIR_FP64 / PEAK_FP64  # s per pair at peak
+# Algorithmic work of the quad kernels (one sin/cos per quad of pixels
+# (+-x, +-y, +-z), the ring factor once per 16-quad tile), per unit counted by
+# viss_sky_count_work with the kernels' own occultation test; sin + cos count
+# 20 FP32 flops as in the per-pair convention, an FMA 2:
+#   (row, quad) with a visible pixel:   phase u x + v y (mul + FMA: 3 FP64) and its
+#                                       mod-1 reduction (1 FP64), sin + cos (20 FP32)
+#   (row, quad, ring) with a visible pixel: the ring's two multiply-adds (4 FP32)
+#   (row, tile) with a visible pixel:   ring phase w z and its reduction (2 FP64), sin + cos
+#                                       (20 FP32), two complex products into the row sum (12 FP32)
+#   (row, rest pair) with a visible pixel: phase (5 FP64) + reduction (1), sin + cos (20),
+#                                       two multiply-adds (4 FP32)
+# The occultation classification and per-pixel tests are not counted: they decide
+# which of these units run, and their cost shows in the VALU issue fraction.
+QUAD_WORK = {"row_quads": (4, 20), "row_rings": (0, 4), "row_tiles": (2, 32), "row_rest_pairs": (6, 24)}
+
+
+def quad_flops(work):
+    """(FP64 flops, FP32 flops) of one pass of the quad kernels over counted work."""
+    f64 = sum(work[k] * QUAD_WORK[k][0] for k in QUAD_WORK)
+    f32 = sum(work[k] * QUAD_WORK[k][1] for k in QUAD_WORK)
+    return f64, f32
 
 
 def log(*a):
@@ -65,6 +88,9 @@ def parse():
                     help="run only the batched comparison (profiling runs of that workload)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the HBM-traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
+    ap.add_argument("--summarize-valu", metavar="PMC_DIR",
+                    help="print the VALU issue fractions of a rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_VALU_TRANS_F32 "
+                         "GRBM_GUI_ACTIVE run and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -126,10 +152,53 @@ def committed_traffic(kernel, workload):
     return None
 
 
+def committed_valu_issue(kernel, workload):
+    """valu_issue_frac of `kernel` from profiles/valu_issue_<workload>.txt (None if absent)."""
+    path = os.path.join(ROOT, "profiles", f"valu_issue_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"kernel={kernel} "):
+            for tok in line.split():
+                if tok.startswith("valu_issue_frac="):
+                    return float(tok.split("=")[1])
+    return None
+
+
+def summarize_valu(d):
+    """VALU issue fraction per kernel from a rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_VALU_TRANS_F32
+    GRBM_GUI_ACTIVE run: a wavefront VALU instruction issues in 4 cycles on one of the 1024 SIMDs
+    (256 CUs x 4), so frac = 4 SQ_INSTS_VALU / (1024 cycles); GRBM_GUI_ACTIVE is summed over the
+    8 XCDs in the CSV.  Sin / cos issue in 8 cycles: the weighted fraction counts them twice."""
+    rows = _pmc_rows(d)
+    by = {}
+    for r in rows:
+        key = (r["Dispatch_Id"], r["Kernel_Name"], r["Grid_Size"])
+        by.setdefault(key, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    out = []
+    seen = set()
+    for (did, name, grid), c in sorted(by.items(), key=lambda kv: int(kv[0][0])):
+        short = name.split("(")[0].replace("void ", "").split("<")[0].split("::")[-1]
+        if (short, grid) in seen or "GRBM_GUI_ACTIVE" not in c or "SQ_INSTS_VALU" not in c:
+            continue
+        if not short.startswith("k_") or c["GRBM_GUI_ACTIVE"] < 8e6:  # kernels of a millisecond and more
+            continue
+        seen.add((short, grid))
+        cyc = c["GRBM_GUI_ACTIVE"] / 8.0
+        valu, trans = c["SQ_INSTS_VALU"], c.get("SQ_INSTS_VALU_TRANS_F32", 0.0)
+        out.append(f"kernel={short} valu_issue_frac={4 * valu / (1024 * cyc):.4f} "
+                   f"valu_issue_frac_trans8={4 * (valu + trans) / (1024 * cyc):.4f} sq_insts_valu={valu:.6e} "
+                   f"sq_insts_valu_trans_f32={trans:.6e} gpu_cycles={cyc:.6e} grid={grid}")
+    return "\n".join(out)
+
+
 def main():
     args = parse()
     if args.summarize_pmc:
         print(summarize_pmc(*args.summarize_pmc))
+        return
+    if args.summarize_valu:
+        print(summarize_valu(args.summarize_valu))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
@@ -231,21 +300,36 @@ def main():
     kf, kb = (("k_forward_quads", "k_backproject_quads") if quads else
               ("k_forward_pairs", "k_backproject_pairs") if paired else ("k_forward", "k_backproject"))
     dom, dom_ms = (kf, fwd_avg) if fwd_avg >= adj_avg else (kb, adj_avg)
-    flops = visible * (PAIR_FP32 + PAIR_FP64)
-    achieved = flops / (dom_ms * 1e-3)
-    peak = (PAIR_FP32 + PAIR_FP64) / PAIR_T_PEAK
-    out["roofline"] = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
-                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": committed_traffic(dom, wl.name),
-                       "flops_per_pair": {"fp32": PAIR_FP32, "fp64": PAIR_FP64}, "pairs_per_launch": visible}
     if quads:
-        out["roofline"]["note"] = ("flops are the frozen per-(visibility, pixel)-pair count; the quad kernels "
-                                   "evaluate one sin/cos per four pixels (x, y, +-z), (-x, -y, +-z) (HEALPix is "
-                                   "symmetric under s -> -s and z -> -z), so they execute fewer flops than "
-                                   "counted and frac can exceed 1")
-    elif paired:
-        out["roofline"]["note"] = ("flops are the frozen per-(visibility, pixel)-pair count; the antipodal-pair "
-                                   "kernels evaluate one phase and one sin/cos per pixel pair (s, -s), so they "
-                                   "execute fewer flops than counted and frac can exceed 1")
+        work = sky.count_work(xa, xb, R_moon=R)
+        assert work["visible_pairs"] == visible
+        f64, f32 = quad_flops(work)
+        t_peak = f32 / PEAK_FP32 + f64 / PEAK_FP64
+        out["roofline"] = {
+            "bound": "valu", "kernel": dom, "achieved": (f32 + f64) / (dom_ms * 1e-3) / 1e12,
+            "peak": (f32 + f64) / t_peak / 1e12, "unit": "TFLOP/s", "frac": t_peak / (dom_ms * 1e-3),
+            "traffic": committed_traffic(dom, wl.name), "kernel_ms": dom_ms,
+            "flops_per_launch": {"fp64": f64, "fp32": f32}, "work_per_launch": work,
+            "flops_per_unit": {k: {"fp64": a, "fp32": b} for k, (a, b) in QUAD_WORK.items()},
+            "note": ("algorithmic flops of the quad kernels that run (units counted exactly by "
+                     "viss_sky_count_work); peak = the blended FP32 / FP64 vector peak for that mix; "
+                     "frac = time at peak / kernel time"),
+        }
+        issue = committed_valu_issue(dom, wl.name)
+        if issue is not None:
+            out["roofline"]["valu_issue_frac"] = issue
+    else:
+        out["roofline"] = {"bound": "valu", "kernel": dom, "achieved": None, "peak": None, "unit": "TFLOP/s",
+                           "frac": None, "traffic": committed_traffic(dom, wl.name),
+                           "note": "the sky has no quad layout: no algorithmic count for its kernels"}
+    # the frozen per-(visibility, pixel)-pair convention, as an equivalent rate
+    eq = visible * (PAIR_FP32 + PAIR_FP64) / (dom_ms * 1e-3)
+    out["roofline"]["per_pair_equivalent"] = {
+        "achieved": eq / 1e12, "peak": (PAIR_FP32 + PAIR_FP64) / PAIR_T_PEAK / 1e12, "unit": "TFLOP/s",
+        "ratio": eq * PAIR_T_PEAK / (PAIR_FP32 + PAIR_FP64), "pairs_per_launch": visible,
+        "flops_per_pair": {"fp32": PAIR_FP32, "fp64": PAIR_FP64},
+        "note": ("the frozen count charges a sin/cos to every visible (visibility, pixel) pair; the quad kernels "
+                 "share one sin/cos among four pixels, so this ratio is not a fraction of peak and exceeds 1")}
     out["detail"] = {
         "visible_pairs_per_pass": visible, "dense_pairs_per_pass": dense, "visible_fraction": visible / dense,
         "fwd_ms": fwd_avg, "adj_ms": adj_avg, "fwd_pairs_per_s": visible / (fwd_avg * 1e-3),
@@ -316,7 +400,8 @@ def main():
     log(f"[bench] snapshot forward: {sf_ms:.1f} ms ({fwd_avg / sf_ms:.2f}x the per-row forward)")
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R)
+        out["cpu_baseline"], out["oracle_5pct"] = cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa,
+                                                               xb, R)
 
     if not args.no_extras:
         out["hbm_passes"] = hbm_passes(ops, inp, wl, dev)
@@ -330,51 +415,82 @@ def main():
 
 
 def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
-    """FP64 oracle (OpenMP on the host cores) on a bounded visibility sample."""
+    """The FP64 oracle (OpenMP on the host cores) on a fixed 5% of the visibility rows.
+
+    The precision workload's accuracy is stated on 5% of its rows (45,000 of 900,000):
+    the mixed V against the oracle's V on those rows, and the mixed density-compensated
+    image of those rows (weights from all rows, the mixed V) against the oracle's image
+    (same weights, the oracle's V).  The rows run in chunks (the image is a sum over
+    rows); the first chunk, sized to about args.cpu_seconds of oracle time, is the timed
+    throughput sample of cpu_baseline."""
     import torch
 
     import oracle
+    from viss_optimize_3d_amd import ops
 
     oracle.build()
     l, m, n = (lmn[i].cpu().numpy() for i in range(3))
     Ih = I.cpu().numpy()
     K = inp.uvw.shape[1]
     rng = np.random.default_rng(123)
-
-    def run(rows):
-        u, a, b = inp.uvw[:, rows], inp.xyz_a[:, rows], inp.xyz_b[:, rows]
-        t0 = time.perf_counter()
-        Vs = oracle.forward(l, m, n, Ih, u, a, b, flags=oracle.MASK, R_moon=R)
-        t1 = time.perf_counter()
-        return Vs, t1 - t0
-
-    # grow the sample until the forward alone takes about half the CPU budget
-    n_rows = 64
+    n5 = max(1, K // 20)
+    rows5 = np.sort(rng.choice(K, n5, replace=False))
+    # density-compensation weights over every row (the shell counts need all of them)
+    bll32 = inp.bll.astype(np.float32)
+    nr = int(np.ceil((float(bll32.max()) - 0.25) / 0.5)) + 1
+    wt = ops.weights(torch.from_numpy(bll32).to(uvw.device), uvw[2].contiguous(), nr, want_dg=False)["wt"]
+    wt5 = wt[torch.from_numpy(rows5).to(uvw.device)].cpu().numpy()
+    # first chunk: grow a probe until its forward takes about a quarter of the sample budget
+    probe = 256
     while True:
-        rows = np.sort(rng.choice(K, n_rows, replace=False))
-        Vs, tf = run(rows)
-        log(f"[bench] cpu baseline: {n_rows} rows forward in {tf:.2f} s")
-        if tf >= args.cpu_seconds / 2 or n_rows >= K:
+        r = rows5[:probe]
+        t0 = time.perf_counter()
+        oracle.forward(l, m, n, Ih, inp.uvw[:, r], inp.xyz_a[:, r], inp.xyz_b[:, r], flags=oracle.MASK, R_moon=R)
+        tp = time.perf_counter() - t0
+        if tp >= args.cpu_seconds / 16 or probe >= n5:
             break
-        n_rows = int(min(K, n_rows * max(2.0, min(8.0, (args.cpu_seconds / 2) / max(tf, 1e-3)))))
-    # adjoint on the same rows (unit weights) with random y
-    ys = rng.normal(size=n_rows) + 1j * rng.normal(size=n_rows)
-    t0 = time.perf_counter()
-    xs = oracle.backproject(l, m, n, inp.uvw[:, rows], ys, None, inp.xyz_a[:, rows], inp.xyz_b[:, rows],
-                            flags=oracle.MASK, R_moon=R)
-    ta = time.perf_counter() - t0
-    # exact visible-pair count of the sample, from the GPU counter
-    rr = torch.from_numpy(rows).to(uvw.device)
-    vis_s = int(sky.count_visible(xa[:, rr].contiguous(), xb[:, rr].contiguous(), R_moon=R).item())
-    Vg = V_mixed[rr].cpu().numpy()
-    rel = float(np.linalg.norm(Vg - Vs) / np.linalg.norm(Vs))
-    mx = float(np.max(np.abs(Vg - Vs)) / np.max(np.abs(Vs)))
-    # mixed GPU adjoint on the same sample for the adjoint's error vs oracle
-    xg = sky.backproject(uvw[:, rr].contiguous(), torch.from_numpy(ys).to(uvw.device), None,
-                         xa[:, rr].contiguous(), xb[:, rr].contiguous(), mask=True, R_moon=R).cpu().numpy()
-    rel_x = float(np.linalg.norm(xg - xs) / np.linalg.norm(xs))
-    mx_x = float(np.max(np.abs(xg - xs)) / np.max(np.abs(xs)))
+        probe = min(n5, probe * 4)
+    first = int(min(n5, max(probe, probe * (args.cpu_seconds / 2) / max(tp, 1e-3))))
+    log(f"[bench] oracle 5%: {n5} rows, first chunk {first} rows (probe {probe} rows forward in {tp:.2f} s)")
+    Vo = np.empty(n5, np.complex128)
+    img_o = np.zeros(l.size)
+    t_fwd = t_adj = 0.0
+    sample = None
+    bounds = [0, first] + list(range(first + 10_000, n5, 10_000)) + [n5]
+    bounds = sorted(set(b for b in bounds if b <= n5))
+    for c0, c1 in zip(bounds[:-1], bounds[1:]):
+        r = rows5[c0:c1]
+        u, a, b = inp.uvw[:, r], inp.xyz_a[:, r], inp.xyz_b[:, r]
+        t0 = time.perf_counter()
+        Vo[c0:c1] = oracle.forward(l, m, n, Ih, u, a, b, flags=oracle.MASK, R_moon=R)
+        t1 = time.perf_counter()
+        img_o += oracle.backproject(l, m, n, u, Vo[c0:c1], wt5[c0:c1], a, b, flags=oracle.MASK, R_moon=R)
+        t2 = time.perf_counter()
+        t_fwd += t1 - t0
+        t_adj += t2 - t1
+        if sample is None:
+            sample = (c1 - c0, t1 - t0, t2 - t1)
+        log(f"[bench] oracle 5%: rows {c1}/{n5} done ({t_fwd + t_adj:.1f} s)")
+    rr = torch.from_numpy(rows5).to(uvw.device)
+    Vg = V_mixed[rr]
+    Vgh = Vg.cpu().numpy()
+    sub = lambda t: t[:, rr].contiguous()  # noqa: E731
+    img_g = sky.backproject(sub(uvw), Vg, wt[rr].contiguous(), sub(xa), sub(xb), mask=True, R_moon=R).cpu().numpy()
+    acc = {"rows": int(n5), "fraction_of_rows": n5 / K,
+           "rel_l2_forward": float(np.linalg.norm(Vgh - Vo) / np.linalg.norm(Vo)),
+           "max_err_forward_rel_to_max": float(np.max(np.abs(Vgh - Vo)) / np.max(np.abs(Vo))),
+           "rel_l2_image": float(np.linalg.norm(img_g - img_o) / np.linalg.norm(img_o)),
+           "max_err_image_rel_to_max": float(np.max(np.abs(img_g - img_o)) / np.max(np.abs(img_o))),
+           "image": "density-compensated back-projection of the 5% rows (weights over all rows): mixed V and "
+                    "kernels against the oracle's V and back-projection", "oracle_seconds": t_fwd + t_adj}
+    log(f"[bench] oracle 5%: V rel L2 {acc['rel_l2_forward']:.2e} (max {acc['max_err_forward_rel_to_max']:.2e}), "
+        f"image rel L2 {acc['rel_l2_image']:.2e} (max {acc['max_err_image_rel_to_max']:.2e})")
+    n_s, tf, ta = sample
+    rs = torch.from_numpy(rows5[:n_s]).to(uvw.device)
+    vis_s = int(sky.count_visible(xa[:, rs].contiguous(), xb[:, rs].contiguous(), R_moon=R).item())
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rows = rows5[:n_s]
+    n_rows = n_s
     # the reference's own FP32 forward (acos mask at the fixed cut angle, phase
     # against n - 1; vissgen3d_noblockage.cu:76-102) on a quarter of the rows,
     # and its error against the FP64 oracle of the same formulation
@@ -404,12 +520,11 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
     numpy1 = {"pairs_per_s": in1.uvw.shape[1] * l1.size / tn, "seconds": tn, "rows": int(in1.uvw.shape[1]),
               "pixels": int(l1.size), "rel_l2_vs_oracle": float(np.linalg.norm(Vn - Vo1) / np.linalg.norm(Vo1))}
     log(f"[bench] cpu NumPy notebook-style forward, {w1.name}: {tn:.2f} s, {numpy1['pairs_per_s']:.3g} pairs/s")
-    return {"value": vis_s / (tf + ta), "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + adjoint "
-                      f"(OpenMP), {tf + ta:.1f} s",
-            "rel_l2_forward_vs_oracle": rel, "max_err_forward_vs_oracle": mx, "rel_l2_adjoint_vs_oracle": rel_x,
-            "max_err_adjoint_vs_oracle": mx_x,
+    base = {"value": vis_s / (tf + ta), "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"{n_rows} of {K} visibility rows x all {l.size} pixels, FP64 oracle forward + "
+                      f"density-compensated back-projection (OpenMP), {tf + ta:.1f} s",
             "reference_fp32_forward": ref32, "numpy_notebook_forward": numpy1}
+    return base, acc
 
 
 def batched(ops, geo, dev):
@@ -448,20 +563,29 @@ def batched(ops, geo, dev):
         return (time.perf_counter() - t0) / reps
 
     t_group = timed(per_group)
+    V.zero_()
+    per_group()
+    V_group = V.clone()
     t_default = timed(lambda: batch.forward(I, out=V))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch.autotune(I, reps=1)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t0
+    side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
         t_graph = timed(lambda: batch.forward(I, out=V, use_graph=True))
+        V.zero_()
+        batch.forward(I, out=V, use_graph=True)
     torch.cuda.synchronize()
+    rel = float(torch.linalg.vector_norm(V - V_group) / torch.linalg.vector_norm(V_group))
     st = batch.stats()
+    log(f"[bench] batched: graph-replayed tuned batch vs per-group launches rel L2 {rel:.2e}")
     return {"groups": G, "rows": int(U.shape[1]), "npix": sky.P, "visible_pairs": visible,
             "visible_tile_fraction": st["visible_tiles"] / st["total_tiles"],
             "per_group_launches_pairs_per_s": visible / t_group, "batched_default_pairs_per_s": visible / t_default,
             "batched_tuned_graph_pairs_per_s": visible / t_graph, "speedup_batched_over_per_group": t_group / t_graph,
+            "rel_l2_batched_vs_per_group": rel,
             "autotune_s": t_tune, "work_items": st["n_items"], "work_items_untuned": st0["n_items"]}
 
 

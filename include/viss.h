@@ -176,6 +176,19 @@ int viss_sky_backproject_noalloc(viss_sky* sky, const double* uvw, const double*
 int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,
                            const viss_mask_geometry* geom, int64_t* count, void* stream);
 
+/* Work of the mixed-precision per-row kernels over a sky's quad layout for K
+ * masked rows, counted with the kernels' own occultation test; accumulated
+ * into counts (device int64 [5]):
+ *   [0] visible (row, pixel) pairs (= viss_sky_count_visible),
+ *   [1] (row, 16-quad tile) with a visible pixel: one ring factor each,
+ *   [2] (row, quad) with a visible pixel: one phase and one sin/cos each,
+ *   [3] (row, quad, ring z or -z) with a visible pixel: one pair of FMAs each,
+ *   [4] (row, pair outside the quads) with a visible pixel: one phase and sin/cos.
+ * The measurement of the kernels' algorithmic work (no counterpart in the
+ * reference).  VISS_ERR_INVALID for a sky without the quad layout. */
+int viss_sky_count_work(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,
+                        const viss_mask_geometry* geom, int64_t* counts, void* stream);
+
 /* ---- batched forward: many visibility groups (snapshot groups x
  * frequencies) over one sky in ONE launch.  Replaces the
  * reference's per-day loop of forward launches (vissgen3d_noblockage.cu:

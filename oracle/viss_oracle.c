@@ -25,6 +25,9 @@
  *       closed form, mpmath brute force.
  *   mask_flips  : its dot side equals count_visible, its acos side the count
  *       of forward_ref32 with unit brightness and zero baselines.
+ *   mask_threshold / visible : the same expression in exact rational
+ *       arithmetic on pairs placed on the cut, where the evaluation order
+ *       decides (test_oracle_cut.py).
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  */
@@ -172,18 +175,21 @@ void oracle_pixel_dirs(const double* theta, const double* phi, const double* B, 
  * (orbit_radius = R+h); orbit_radius <= 0 uses each satellite's own |r|:
  * t = -sqrt(|r|^2 - R^2). */
 double oracle_mask_threshold(double x, double y, double z, double R_moon, double orbit_radius) {
-  const double r2 = x * x + y * y + z * z;
+  /* Explicit fma: the decision at the cut is an exact comparison, so t and
+   * s.r are evaluated in one fixed order (the GPU kernels use the same). */
+  const double r2 = fma(x, x, fma(y, y, z * z));
   if (orbit_radius > 0.0) {
     const double q = R_moon / orbit_radius;
-    const double c = -sqrt(1.0 - q * q);
+    const double c = -sqrt(fma(-q, q, 1.0));
     return c * sqrt(r2);
   }
-  const double d = r2 - R_moon * R_moon;
+  const double d = fma(-R_moon, R_moon, r2);
   return d > 0.0 ? -sqrt(d) : 0.0; /* satellite inside the moon: sees the upper half-space only */
 }
 
+/* s.r >= t, with s.r = fma(l, x, fma(m, y, n * z)). */
 static inline int visible(double l, double m, double n, double x, double y, double z, double t) {
-  return ((l * x + m * y) + n * z) >= t;
+  return fma(l, x, fma(m, y, n * z)) >= t;
 }
 
 /* Forward direct sum, healpix_moonback_viss (vissgen3d_noblockage.cu:76-102):

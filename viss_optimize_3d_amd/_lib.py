@@ -27,7 +27,7 @@ EXPORTS = (
     "viss_version", "viss_status_string", "viss_last_error", "viss_pix2ang", "viss_pixel_dirs", "viss_forward",
     "viss_backproject", "viss_weights", "viss_sky_create", "viss_sky_destroy", "viss_sky_npix", "viss_sky_pairs",
     "viss_sky_quads", "viss_sky_forward",
-    "viss_sky_backproject", "viss_sky_backproject_noalloc", "viss_sky_count_visible", "viss_batch_create", "viss_batch_destroy", "viss_batch_forward",
+    "viss_sky_backproject", "viss_sky_backproject_noalloc", "viss_sky_count_visible", "viss_sky_count_work", "viss_batch_create", "viss_batch_destroy", "viss_batch_forward",
     "viss_batch_autotune", "viss_batch_tuning_save", "viss_batch_tuning_load", "viss_batch_stats",
     "viss_image_accumulate", "viss_reorder", "viss_sky_backproject_snapshots", "viss_sky_forward_snapshots",
 )
@@ -78,6 +78,7 @@ def load():
         "viss_sky_backproject": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _geom_p, _int, _vp, _vp], _int),
         "viss_sky_backproject_noalloc": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _geom_p, _int, _vp, _vp], _int),
         "viss_sky_count_visible": ([_vp, _vp, _vp, _i64, _geom_p, _vp, _vp], _int),
+        "viss_sky_count_work": ([_vp, _vp, _vp, _i64, _geom_p, _vp, _vp], _int),
         "viss_batch_create": ([_vp, _i64, _vp, _vp, _vp, _vp, _geom_p, _int, ctypes.POINTER(_vp), _vp], _int),
         "viss_batch_destroy": ([_vp], _int),
         "viss_batch_forward": ([_vp, _vp, _vp, _int, _vp], _int),

@@ -113,15 +113,26 @@ constexpr int kPadTo = kBpTile;  // sky padding granularity
 __device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 // Occultation threshold t(r): s visible from r iff s.r >= t
-// (vissgen3d_noblockage.cu:88-94,240-243 in dot-product form).
+// (vissgen3d_noblockage.cu:88-94,240-243 in dot-product form).  Written with
+// explicit FMAs so that no compiler contraction changes its rounding: the
+// oracle evaluates the same operations, so both get the same t bit for bit.
 __host__ __device__ inline double mask_threshold(double x, double y, double z, double R_moon, double orbit_radius) {
-  const double r2 = x * x + y * y + z * z;
+  const double r2 = fma(x, x, fma(y, y, z * z));
   if (orbit_radius > 0.0) {
     const double q = R_moon / orbit_radius;
-    return -sqrt(1.0 - q * q) * sqrt(r2);
+    return -sqrt(fma(-q, q, 1.0)) * sqrt(r2);
   }
-  const double d = r2 - R_moon * R_moon;
+  const double d = fma(-R_moon, R_moon, r2);
   return d > 0.0 ? -sqrt(d) : 0.0;
+}
+
+// The occultation test's dot product s.r in the one evaluation order every
+// kernel and the oracle use: fma(sx, rx, fma(sy, ry, sz * rz)).  Kernels that
+// serve the antipode -s of a pixel negate it (exact), and the quad kernels
+// pass sz * rz of the ring (z) or of its mirror ring (-z, whose product is the
+// exact negation).
+__host__ __device__ inline double occ_dot(double sx, double sy, double szrz, double rx, double ry) {
+  return fma(sx, rx, fma(sy, ry, szrz));
 }
 
 // Device buffers growable in stream order.

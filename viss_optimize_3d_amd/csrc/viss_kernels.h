@@ -22,8 +22,8 @@ __device__ __forceinline__ void sincos_rev(float r, float& s, float& c) {
 }
 
 __device__ __forceinline__ bool both_visible(double l, double m, double n, const VisMask& q) {
-  const double da = fma(l, q.xa, fma(m, q.ya, n * q.za));
-  const double db = fma(l, q.xb, fma(m, q.yb, n * q.zb));
+  const double da = occ_dot(l, m, n * q.za, q.xa, q.ya);
+  const double db = occ_dot(l, m, n * q.zb, q.xb, q.yb);
   return (da >= q.ta) & (db >= q.tb);
 }
 
@@ -196,8 +196,8 @@ __device__ __forceinline__ void pair_tile_forward(const PairPack* __restrict__ t
     } else {
       double da = 0.0, db = 0.0;
       if (TA == kSideTest || TB == kSideTest) {
-        da = fma(q.l, mk.xa, fma(q.m, mk.ya, q.n * mk.za));
-        db = fma(q.l, mk.xb, fma(q.m, mk.yb, q.n * mk.zb));
+        da = occ_dot(q.l, q.m, q.n * mk.za, mk.xa, mk.ya);
+        db = occ_dot(q.l, q.m, q.n * mk.zb, mk.xb, mk.yb);
       }
       const float wa = TA == kSideNone ? 0.f
                        : TA == kSideFull ? q.Ia
@@ -321,12 +321,35 @@ __device__ __forceinline__ void forward_pair_tiles(const PairPack* __restrict__ 
 enum RingMode { kRingNone = 0, kRingFF = 1, kRingFN = 2, kRingNF = 3, kRingTF = 4, kRingFT = 5, kRingTest = 6 };
 constexpr int kRingModes = 7;
 
-// Per-lane thresholds of a tile: pixel (x, y, z) passes satellite a when
-// x xa + y ya >= ta - z za = t1a, (-x, -y, z) when x xa + y ya <= -t1a,
-// (x, y, -z) when >= ta + z za = t2a and (-x, -y, -z) when <= -t2a.
-struct QuadThr {
-  double t1a, t1b, t2a, t2b;
+// Occultation tests of a quad in the evaluation order of the per-pixel test
+// (occ_dot).  With E = occ_dot(x, y, z zs) and F = occ_dot(x, y, -z zs) for
+// satellite s, the pixels' dot products are a = (x, y, z): E, b = (-x, -y, z):
+// -F, c = (x, y, -z): F, d = (-x, -y, -z): -E (negation is exact), so each
+// pixel's decision is the one the per-pixel test takes for those coordinates.
+// zz = (z za, z zb) of the tile's ring for the lane's row.
+struct QuadOcc {
+  double zza, zzb;
 };
+
+// Visibility of a quad's pixels a, b, c, d for one row; only what the ring
+// modes MZ (ring z: a, b) and MM (ring -z: c, d) test is evaluated.
+template <int MZ, int MM>
+__device__ __forceinline__ void quad_visible(double x, double y, const VisMask& mk, const QuadOcc& zz, bool& va,
+                                             bool& vb, bool& vc, bool& vd) {
+  constexpr bool needE = MZ == kRingTF || MZ == kRingTest || MM == kRingFT || MM == kRingTest;
+  constexpr bool needF = MZ == kRingFT || MZ == kRingTest || MM == kRingTF || MM == kRingTest;
+  va = vb = vc = vd = false;
+  if (needE) {
+    const double ea = occ_dot(x, y, zz.zza, mk.xa, mk.ya), eb = occ_dot(x, y, zz.zzb, mk.xb, mk.yb);
+    va = (ea >= mk.ta) & (eb >= mk.tb);
+    vd = (-ea >= mk.ta) & (-eb >= mk.tb);
+  }
+  if (needF) {
+    const double fa = occ_dot(x, y, -zz.zza, mk.xa, mk.ya), fb = occ_dot(x, y, -zz.zzb, mk.xb, mk.yb);
+    vc = (fa >= mk.ta) & (fb >= mk.tb);
+    vb = (-fa >= mk.ta) & (-fb >= mk.tb);
+  }
+}
 
 template <int MR>
 __device__ __forceinline__ void ring_step(float w1, float w2, float sum, float dif, bool v1, bool v2, float cs,
@@ -357,22 +380,17 @@ __device__ __forceinline__ void ring_step(float w1, float w2, float sum, float d
 
 template <int MZ, int MM>
 __device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ tq, double u, double v,
-                                                  const VisMask& mk, const QuadThr& th, float& Rz, float& Jz,
+                                                  const VisMask& mk, const QuadOcc& zz, float& Rz, float& Jz,
                                                   float& Rm, float& Jm) {
 #pragma unroll 4
   for (int j = 0; j < kQuadTile; ++j) {
     const QuadPack q = tq[j];
     float sn, cs;
     sincos_rev(rev_fp32(fma(u, q.x, v * q.y)), sn, cs);
-    double pa = 0.0, pb = 0.0;
-    if (MZ >= kRingTF || MM >= kRingTF) {
-      pa = fma(q.x, mk.xa, q.y * mk.ya);
-      pb = fma(q.x, mk.xb, q.y * mk.yb);
-    }
-    ring_step<MZ>(q.Ia, q.Ib, q.Az, q.Dz, (pa >= th.t1a) & (pb >= th.t1b), (pa <= -th.t1a) & (pb <= -th.t1b), cs,
-                  sn, Rz, Jz);
-    ring_step<MM>(q.Ic, q.Id, q.Am, q.Dm, (pa >= th.t2a) & (pb >= th.t2b), (pa <= -th.t2a) & (pb <= -th.t2b), cs,
-                  sn, Rm, Jm);
+    bool va, vb, vc, vd;
+    quad_visible<MZ, MM>(q.x, q.y, mk, zz, va, vb, vc, vd);
+    ring_step<MZ>(q.Ia, q.Ib, q.Az, q.Dz, va, vb, cs, sn, Rz, Jz);
+    ring_step<MM>(q.Ic, q.Id, q.Am, q.Dm, vc, vd, cs, sn, Rm, Jm);
   }
 }
 
@@ -414,24 +432,21 @@ __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ 
   for (int64_t t = t0; t < t1; ++t) {
     const double z = qz[t];
     int mz = kRingFF, mm = kRingFF;
-    QuadThr th{0.0, 0.0, 0.0, 0.0};
+    QuadOcc zz{0.0, 0.0};
     if (MASK) {
       const PairSides ad = classify_pair_wave(qcone[2 * t], r.cl, actm);      // a and its antipode d
       const PairSides bc = classify_pair_wave(qcone[2 * t + 1], r.cl, actm);  // b and its antipode c
       mz = ring_mode(ad.seeA, ad.wholeA, bc.seeA, bc.wholeA);
       mm = ring_mode(bc.seeB, bc.wholeB, ad.seeB, ad.wholeB);
       if (mz == kRingNone && mm == kRingNone) continue;
-      const double za = z * r.mk.za, zb = z * r.mk.zb;
-      th.t1a = r.mk.ta - za;
-      th.t1b = r.mk.tb - zb;
-      th.t2a = r.mk.ta + za;
-      th.t2b = r.mk.tb + zb;
+      zz.zza = z * r.mk.za;
+      zz.zzb = z * r.mk.zb;
     }
     const QuadPack* __restrict__ tq = qp + t * kQuadTile;
     float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
 #define VISS_QT(A, B)                                                                      \
   case A * kRingModes + B:                                                                 \
-    quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, th, Rz, Jz, Rm, Jm);                      \
+    quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, zz, Rz, Jz, Rm, Jm);                      \
     break;
 #define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
                        VISS_QT(A, kRingTF) VISS_QT(A, kRingFT) VISS_QT(A, kRingTest)

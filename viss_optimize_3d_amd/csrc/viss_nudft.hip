@@ -328,8 +328,8 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
       const VisMask mk = vm[kb + j];
       float sn, cs;
       sincos_rev(rev_fp32(fma(h.u, l, fma(h.v, m, h.w * n))), sn, cs);
-      const double da = fma(l, mk.xa, fma(m, mk.ya, n * mk.za));
-      const double db = fma(l, mk.xb, fma(m, mk.yb, n * mk.zb));
+      const double da = occ_dot(l, m, n * mk.za, mk.xa, mk.ya);
+      const double db = occ_dot(l, m, n * mk.zb, mk.xb, mk.yb);
       const float ta = fmaf(h.yr, cs, -h.yi * sn), tb = fmaf(h.yr, cs, h.yi * sn);
       sa += ((da >= mk.ta) & (db >= mk.tb)) ? ta : 0.f;
       sb += ((-da >= mk.ta) & (-db >= mk.tb)) ? tb : 0.f;
@@ -392,13 +392,7 @@ __device__ __forceinline__ void quad_bp_rows(uint64_t msk, const VisHot* __restr
     bool va = false, vb = false, vc = false, vd = false;
     if (MZ >= kRingTF || MM >= kRingTF) {  // unused tests are dropped by the compiler
       const VisMask mk = vm[kb + j];
-      const double pa = fma(qx, mk.xa, qy * mk.ya), pb = fma(qx, mk.xb, qy * mk.yb);
-      const double za = zl * mk.za, zb = zl * mk.zb;
-      const double t1a = mk.ta - za, t1b = mk.tb - zb, t2a = mk.ta + za, t2b = mk.tb + zb;
-      va = (pa >= t1a) & (pb >= t1b);
-      vb = (pa <= -t1a) & (pb <= -t1b);
-      vc = (pa >= t2a) & (pb >= t2b);
-      vd = (pa <= -t2a) & (pb <= -t2b);
+      quad_visible<MZ, MM>(qx, qy, mk, QuadOcc{zl * mk.za, zl * mk.zb}, va, vb, vc, vd);
     }
     ring_bp<MZ>(r.x, r.y, cs, sn, va, vb, acc.Pz, acc.Qz, acc.Az, acc.Bz);
     ring_bp<MM>(r.z, r.w, cs, sn, vc, vd, acc.Pm, acc.Qm, acc.Cm, acc.Dm);
@@ -547,6 +541,73 @@ __global__ __launch_bounds__(256) void k_count_visible(const PixPack* __restrict
     }
   }
   if (act && c) atomicAdd(count, c);
+}
+
+// ------------------------------------------------------------ work count of the quad layout
+// The work of the mixed per-row kernels over the quad layout, counted with
+// their own occultation test (quad_visible, occ_dot): per row, the visible
+// pixels, the quad tiles with a visible pixel (one ring factor each), the
+// quads with a visible pixel (one phase and one sin/cos each), the rings of a
+// quad with a visible pixel (one pair of multiply-adds each), and the pairs of
+// the rest layout with a visible pixel (one phase and sin/cos each).  One row
+// per lane; grid.y chunks of quad tiles, then of rest tiles.
+__global__ __launch_bounds__(256) void k_count_quad_work(
+    const QuadPack* __restrict__ qp, const int4* __restrict__ qi, const double* __restrict__ qz, int64_t nqtile,
+    int64_t qpc, int64_t nqchunk, const PairPack* __restrict__ rp, const int2* __restrict__ ri, int64_t nrtile,
+    int64_t rpc, const double* __restrict__ xyza, const double* __restrict__ xyzb, int64_t K, double R_moon,
+    double orbit_r, unsigned long long* __restrict__ counts) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool act = k < K;
+  const int64_t kk = act ? k : K - 1;
+  VisMask mk;
+  VisCls cl;
+  make_mask_records(xyza[kk], xyza[K + kk], xyza[2 * K + kk], xyzb[kk], xyzb[K + kk], xyzb[2 * K + kk], R_moon,
+                    orbit_r, mk, cl);
+  unsigned long long npix = 0, ntile = 0, nquad = 0, nring = 0, npair = 0;
+  const int64_t c = blockIdx.y;
+  if (c < nqchunk) {
+    const int64_t t1 = min(nqtile, (c + 1) * qpc);
+    for (int64_t t = c * qpc; t < t1; ++t) {
+      const double z = qz[t];
+      const QuadOcc zz{z * mk.za, z * mk.zb};
+      unsigned long long tq = 0;
+      for (int j = 0; j < kQuadTile; ++j) {
+        const QuadPack q = qp[t * kQuadTile + j];
+        const int4 id = qi[t * kQuadTile + j];
+        bool va, vb, vc, vd;
+        quad_visible<kRingTest, kRingTest>(q.x, q.y, mk, zz, va, vb, vc, vd);
+        va &= id.x >= 0;
+        vb &= id.y >= 0;
+        vc &= id.z >= 0;
+        vd &= id.w >= 0;
+        npix += (unsigned)va + (unsigned)vb + (unsigned)vc + (unsigned)vd;
+        nring += (unsigned)(va | vb) + (unsigned)(vc | vd);
+        const bool any = va | vb | vc | vd;
+        nquad += any;
+        tq += any;
+      }
+      ntile += tq > 0;
+    }
+  } else {
+    const int64_t r0 = (c - nqchunk) * rpc, r1 = min(nrtile, r0 + rpc);
+    for (int64_t e = r0 * 64; e < r1 * 64; ++e) {
+      const PairPack q = rp[e];
+      const int2 id = ri[e];
+      const double da = occ_dot(q.l, q.m, q.n * mk.za, mk.xa, mk.ya);
+      const double db = occ_dot(q.l, q.m, q.n * mk.zb, mk.xb, mk.yb);
+      const bool va = (id.x >= 0) & (da >= mk.ta) & (db >= mk.tb);
+      const bool vb = (id.y >= 0) & (-da >= mk.ta) & (-db >= mk.tb);
+      npix += (unsigned)va + (unsigned)vb;
+      npair += va | vb;
+    }
+  }
+  if (!act) npix = ntile = nquad = nring = npair = 0;
+  unsigned long long v[5] = {npix, ntile, nquad, nring, npair};
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o, 64);
+    if ((threadIdx.x & 63) == 0 && v[i]) atomicAdd(&counts[i], v[i]);
+  }
 }
 
 // ------------------------------------------------------------ launch helpers
@@ -860,6 +921,30 @@ int viss_backproject(const double* lmn, int64_t P, const double* uvw, const doub
   rc = sky_backproject_impl(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, as_stream(stream));
   viss_sky_destroy(sky);
   return rc;
+}
+
+int viss_sky_count_work(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,
+                        const viss_mask_geometry* geom, int64_t* counts, void* stream) {
+  VISS_REQUIRE(sky != nullptr, "viss_sky_count_work: sky is NULL");
+  int rc = check_device(sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE(K >= 0, "viss_sky_count_work: negative K");
+  VISS_REQUIRE(sky->use_quads, "viss_sky_count_work: the sky has no quad layout (its kernels are not the quad kernels)");
+  if (K == 0) return VISS_OK;
+  VISS_REQUIRE(xyz_a && xyz_b && counts && geom, "viss_sky_count_work: NULL argument");
+  rc = validate_geom(geom, VISS_MASK);
+  if (rc != VISS_OK) return rc;
+  const int64_t bx = cdiv(K, 256);
+  VISS_REQUIRE(bx <= INT32_MAX, "viss_sky_count_work: K too large for one launch");
+  const int64_t qpc = std::max<int64_t>(64, cdiv(sky->nqtile, 32768)), rpc = std::max<int64_t>(8, cdiv(sky->nrtile, 32768));
+  const int64_t nqc = cdiv(sky->nqtile, qpc), nrc = cdiv(sky->nrtile, rpc);
+  dim3 grid((unsigned)bx, (unsigned)(nqc + nrc));
+  k_count_quad_work<<<grid, 256, 0, as_stream(stream)>>>(sky->qpack, sky->qidx, sky->qz, sky->nqtile, qpc, nqc,
+                                                         sky->rpack, sky->ridx, sky->nrtile, rpc, xyz_a, xyz_b, K,
+                                                         geom->R_moon, geom->orbit_radius,
+                                                         (unsigned long long*)counts);
+  VISS_LAUNCHED(k_count_quad_work);
+  return VISS_OK;
 }
 
 int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,

@@ -141,7 +141,7 @@ __device__ __forceinline__ void snapshot_phasors(const SatRec* __restrict__ R, c
     const SatRec q = R[i];
 #pragma unroll
     for (int p = 0; p < PPL; ++p) {
-      const double D = fma(q.x, l[p], fma(q.y, m[p], q.z * n[p]));
+      const double D = occ_dot(l[p], m[p], n[p] * q.z, q.x, q.y);
       float sn, cs;
       sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
       const bool vis = !VIS || (D >= q.t);
@@ -421,7 +421,7 @@ __device__ __forceinline__ void pair_phasors(const SatRec* __restrict__ R, doubl
 #pragma unroll
   for (int i = 0; i < NMAX; ++i) {
     const SatRec q = R[i];
-    const double D = fma(q.x, l, fma(q.y, m, q.z * n));
+    const double D = occ_dot(l, m, n * q.z, q.x, q.y);
     float sn, cs;
     sincos_rev(rev_fp32(D * inv_lambda), sn, cs);
     gr[i] = cs;
