@@ -75,6 +75,7 @@ int oracle_pix2ang_nest(int64_t nside, double* theta, double* phi, int32_t* ring
   static const int jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
   static const int jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
   if (nside < 1 || nside > (1 << 13)) return -1;
+  if (nside & (nside - 1)) return -1; /* NEST numbers the nside^2 pixels of a face in bit order */
   const int64_t npface = nside * nside, npix = 12 * npface, nl4 = 4 * nside;
   const double three_ns2 = 3.0 * (double)npface;
 #pragma omp parallel for schedule(static)

@@ -86,6 +86,52 @@ int main() {
                      "+-1 9",
                      "long mantissas and specials");
   bad |= check_parse("1 2 3 x 4 5", "stop at a word");
+  // malformed and truncated tokens, huge exponents and digit strings, no trailing newline
+  bad |= check_parse("1e", "exponent without digits");
+  bad |= check_parse("2 1e+", "exponent sign without digits");
+  bad |= check_parse("3 -", "a lone sign");
+  bad |= check_parse("4 .", "a lone point");
+  bad |= check_parse("5 +.e5", "point and exponent without digits");
+  bad |= check_parse("6 1.5e-", "truncated negative exponent");
+  bad |= check_parse("7 1e999999999999999999 -1e-999999999999999999 8", "huge exponents");
+  bad |= check_parse("9 0e123456789 0.0000e-99999", "zero with huge exponents");
+  bad |= check_parse(std::string(1000, '7') + " " + "0." + std::string(800, '0') + "1 " + std::string(400, '9') +
+                         "e-390",
+                     "long digit strings");
+  bad |= check_parse("nan(123) nan(abc) -nan() infinity -INFINITY 1", "nan payloads and spelled infinity");
+  bad |= check_parse("1 2 3", "no trailing newline");
+  bad |= check_parse("\n\n-0.5", "leading blank lines");
+  // fuzz: random strings over the characters numbers are made of
+  {
+    const char alpha[] = "0123456789+-.eEinfaxp \n\t";
+    std::uniform_int_distribution<int> pick(0, (int)sizeof(alpha) - 2), len(0, 40);
+    int fuzz_bad = 0;
+    for (int t = 0; t < 20000 && !fuzz_bad; ++t) {
+      std::string f;
+      const int nt = 1 + t % 7;
+      for (int i = 0; i < nt; ++i) {
+        const int L = len(rng);
+        for (int j = 0; j < L; ++j) f += alpha[pick(rng)];
+        f += (t & 1) ? " " : "\n";
+      }
+      if (t % 3 == 0 && !f.empty()) f.pop_back();  // no trailing separator
+      const std::vector<double> a = vissio::parse_numbers(f), b = parse_ref(f);
+      bool same = a.size() == b.size();
+      for (size_t i = 0; same && i < a.size(); ++i) same = same_bits(a[i], b[i]);
+      if (!same) {
+        printf("FAIL parse fuzz #%d: \"%s\": %zu vs %zu numbers\n", t, f.c_str(), a.size(), b.size());
+        fuzz_bad = 1;
+      }
+    }
+    if (!fuzz_bad) printf("parse fuzz: 20000 random strings identical to strtod\n");
+    bad |= fuzz_bad;
+  }
+  // a large buffer split over threads where a split point falls inside long tokens
+  {
+    std::string big;
+    for (int i = 0; i < 200000; ++i) big += (i % 13 == 0) ? std::string(300, '1') + "e-290 " : "12.5 ";
+    bad |= check_parse(big, "long tokens across thread splits");
+  }
   bad |= check_parse("", "empty");
   bad |= check_parse("   \n\t  ", "whitespace only");
   // writer: bytes as snprintf("%.6g") of the float value, or "%.*g" of the double
