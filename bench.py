@@ -656,9 +656,10 @@ def hbm_passes(ops, inp, wl, dev):
                                                      ops._p(dcf), None, ops._p(wt), None, ops._stream()))
 
         t = timed(fw)
-        # algorithmic bytes per row: read bll 4 and w 8, write gs 4 and wt 8 (FP64); the
-        # implementation's second read of bll and gs (8 B) is traffic, not counted
-        byt = 24 * rows
+        # algorithmic bytes per row (the two-pass convention of the weights: the histogram
+        # pass reads bll, the weight pass reads bll and w and writes gs and wt, since no
+        # weight is known before every row is counted): 4 + 4 + 8 (w, FP64) + 4 + 8 (wt, FP64)
+        byt = 28 * rows
         res[f"weights_rows{rows}"] = {"nr": nr, "bytes": byt, "s": t, "GB_per_s": byt / t / 1e9,
                                       "frac_hbm": byt / t / PEAK_HBM}
         del bll, w, gs, mb, dcf, wt

@@ -6,7 +6,27 @@
  * Conventions (all entry points):
  *   - every array argument is a DEVICE pointer on the current HIP device,
  *     owned by the caller; `stream` is a hipStream_t (NULL = default stream);
- *   - calls are asynchronous on `stream` and never synchronise;
+ *   - calls are asynchronous on `stream`.  These calls synchronise or
+ *     allocate device memory; every other call does neither (it only
+ *     launches kernels on `stream`):
+ *       viss_sky_create        allocates the sky and synchronises the stream
+ *                              (up to 6 times) to size its layouts;
+ *       viss_sky_destroy, viss_batch_destroy   free (hipFree waits for the device);
+ *       viss_forward, viss_backproject         create and destroy a temporary sky per call;
+ *       viss_sky_backproject, viss_sky_forward_snapshots,
+ *       viss_sky_backproject_snapshots         when the sky's row-record buffer must grow:
+ *                              the first such call on a sky and any later call that needs
+ *                              more bytes than any before (more rows, FP64 or masked records):
+ *                              hipStreamSynchronize, hipFree, hipMalloc; otherwise neither;
+ *       viss_batch_create, viss_batch_autotune, viss_batch_tuning_load
+ *                              allocate the work table and synchronise the stream;
+ *       viss_batch_forward with use_graph: on its first call and when I, V or
+ *                              the stream change, it captures and instantiates a graph
+ *                              (host-side objects, no device memory, no synchronisation);
+ *     viss_sky_backproject_noalloc never allocates or synchronises;
+ *   - a sequence without such calls (viss_sky_forward, viss_sky_backproject
+ *     after its buffer is sized, viss_weights, viss_pix2ang, ...) can be
+ *     captured into a HIP graph;
  *   - the return value is a status (VISS_OK = 0, negative on error); the
  *     message of the last failure on the calling thread is viss_last_error();
  *   - "SoA 3xN" means three consecutive blocks of N doubles (x[0..N), y, z);
@@ -85,7 +105,8 @@ int viss_pixel_dirs(const double* theta, const double* phi, const double* B, int
  * lmn: SoA 3xP; I: [P]; uvw: SoA 3xK [wavelengths]; xyz_a/xyz_b: SoA 3xK [m]
  * (only read with VISS_MASK); V: complex [K] ([2K] with VISS_HERMITIAN, rows
  * k+K receiving conj(V_k), vissgen3d_noblockage.cu:100).  geom is read with
- * VISS_MASK.  Builds and frees a temporary sky (see viss_sky_create). */
+ * VISS_MASK.  Builds and frees a temporary sky (see viss_sky_create): allocates
+ * and synchronises. */
 int viss_forward(const double* lmn, const double* I, int64_t P, const double* uvw, const double* xyz_a,
                  const double* xyz_b, int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream);
 
@@ -96,7 +117,8 @@ int viss_forward(const double* lmn, const double* I, int64_t P, const double* uv
  * of viss_gamma_trans, vissgen3d_noblockage.cu:149-151); wt: [K] FP64 or
  * NULL (unit weights: the adjoint A^H of viss_forward); herm = 2 with
  * VISS_HERMITIAN (the K rows are the first half of a Hermitian set), else 1.
- * y: complex [K]; x: [P] FP64. */
+ * y: complex [K]; x: [P] FP64.  Builds and frees a temporary sky: allocates and
+ * synchronises (viss_sky_backproject on a reusable sky does not, once sized). */
 int viss_backproject(const double* lmn, int64_t P, const double* uvw, const double* xyz_a, const double* xyz_b,
                      const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
                      double* x, void* stream);
@@ -131,7 +153,8 @@ int viss_weights(const float* bll, const double* w, int64_t K, int32_t nr, int f
  * {(+-x, +-y, +-z)} (HEALPix is symmetric under z -> -z ring by ring): the
  * mixed-precision per-row forward then evaluates one sin/cos per quad and
  * applies each ring's factor exp(-2 pi i w z) once per tile of 16 quads.
- * Creation synchronises the stream a few times (to size the layouts). */
+ * Creation allocates the sky's arrays and synchronises the stream up to 6
+ * times (to size the layouts); destruction frees them. */
 typedef struct viss_sky viss_sky;
 
 int viss_sky_create(const double* lmn, int64_t P, viss_sky** sky, void* stream);
@@ -147,6 +170,12 @@ int viss_sky_quads(const viss_sky* sky, int64_t* nquad, int64_t* nquad_tiles, in
 
 int viss_sky_forward(viss_sky* sky, const double* I, const double* uvw, const double* xyz_a, const double* xyz_b,
                      int64_t K, const viss_mask_geometry* geom, int flags, double* V, void* stream);
+/* viss_sky_backproject packs per-row records (32 B, 64 B with VISS_PREC_FP64,
+ * plus 96 B with VISS_MASK) into a buffer the sky owns; the call that first
+ * needs more bytes than the buffer holds synchronises `stream`, frees the old
+ * buffer and allocates a larger one (25% headroom).  The row records of one
+ * call are read by that call's kernels only, so calls on one sky must be
+ * ordered (same stream). */
 int viss_sky_backproject(viss_sky* sky, const double* uvw, const double* xyz_a, const double* xyz_b,
                          const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom, int flags,
                          double* x, void* stream);

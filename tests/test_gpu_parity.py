@@ -198,6 +198,10 @@ def test_weights_parity(ops):
     w[7] = 5 * bll[7]  # NaN dg -> weight 1/8
     w[3], w[9], w[10] = bll[3], bll[9], -bll[10]  # |w / bll| = 1 (row 3 in shell nr: dcf 0 with the shift)
     w[11:20] = 0.5 * bll[11:20] * (1.0 + np.array([-1e-9, -1e-12, -1e-15, 0.0, 1e-15, 1e-12, 1e-9, 0.2, -0.2]))
+    # |w / bll| just below 1 (1 - x^2 cancels), and bll of 0, infinity and NaN
+    w[20:25] = bll[20:25] * np.array([1 - 1e-15, 1 - 1e-12, 1 - 1e-9, -(1 - 1e-12), 1 - 1e-6])
+    bll[25:29] = [0.0, 0.0, np.inf, np.nan]
+    w[25:29] = [0.0, 1.0, 7.0, 1.0]
     for shift in (False, True):
         check_weights(ops.weights(cuda(bll), cuda(w), nr, blockage_shift=shift), bll, w, nr, shift)
 

@@ -7,9 +7,13 @@
 //   k_shell_hist : gs_k (FP32, bit-exact with the reference's expression) and
 //                  an LDS-privatised uint32 histogram of gs, flushed with one
 //                  global atomic per non-empty bin per block      [reads 4 B, writes 4 B per row]
-//   k_row_weights: dg_k, wt_k = fmin(dcf[gs_k] dg_k, 1/8), gs_k recomputed from bll,
-//                  and the dcf_j table from the exact shell volumes  [reads 12 B, writes 8-16 B per row]
+//   k_row_weights: the dcf_j table from the exact shell volumes and the counts,
+//                  built per block in LDS (block 0 writes it out; k_dcf_table
+//                  when the table exceeds LDS), then dg_k (one FP64 division),
+//                  wt_k = fmin(dcf[gs_k] dg_k, 1/8) with the table entry gathered
+//                  from LDS, gs_k recomputed from bll [reads 12 B, writes 8-16 B per row]
 // All HBM-bound; the sort is dropped (its order is never used).
+#include <algorithm>
 #include <cstdint>
 
 #include "viss_internal.h"
@@ -41,7 +45,7 @@ __device__ inline void count_shell(int32_t g, int32_t nr, uint32_t* hist, uint32
 // (float4 in, int4 out): the LDS histogram allows only a few blocks per CU,
 // so the wider accesses are what keeps enough bytes in flight.
 template <bool VEC>
-__global__ void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t nr, int32_t* __restrict__ gs,
+__global__ __launch_bounds__(1024) void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t nr, int32_t* __restrict__ gs,
                              uint32_t* __restrict__ mb, unsigned long long* __restrict__ n_bad, int use_lds) {
   extern __shared__ uint32_t hist[];
   if (use_lds) {
@@ -64,12 +68,12 @@ __global__ void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t n
       count_shell(g.w, nr, hist, mb, use_lds, bad);
     };
     int64_t q = t0;
-    for (; q + 3 * stride < K4; q += 4 * stride) {  // four loads in flight before the atomics
-      float4 b[4];
+    for (; q + 7 * stride < K4; q += 8 * stride) {  // eight loads in flight before the atomics
+      float4 b[8];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = b4[q + i * stride];
+      for (int i = 0; i < 8; ++i) b[i] = b4[q + i * stride];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) one(b[i], q + i * stride);
+      for (int i = 0; i < 8; ++i) one(b[i], q + i * stride);
     }
     for (; q < K4; q += stride) one(b4[q], q);
     k_tail = K4 * 4;
@@ -108,60 +112,99 @@ __device__ inline double dcf_value(int64_t j, uint32_t m, int32_t nr, int shift)
   return j == 0 ? 1.0 / (4.0 * kPi) : (jj > nr ? 0.0 : v);
 }
 
-// One row's dg and wt from bll (b), w and the count of its dcf entry (m).
-// gamma = asin(x), x = w / bll: sin(gamma) = x and cos(gamma)^2 = 1 - x^2 on
-// [-1, 1], so dg = 1.5 sqrt(|1 - 4 x^2| / (1 - x^2)), NaN for |x| > 1 (as
-// asin), and no asin or sincos is evaluated.  At |x| = 1 the asin route gives
-// cos(fl(pi/2)) = 6.1e-17, not 0: that value is kept, so dg and wt stay
-// finite there as in the formula.  The row's dcf entry is evaluated as the
-// table's is (dcf_value), so it is the same value.
-__device__ inline void row_weight(float b, double wk, int32_t s, uint32_t m, int32_t nr, int shift, double& dg,
-                                  double& wt) {
-  const double x = wk / (double)b;
-  const double q = (1.0 - x) * (1.0 + x), a = fabs(1.0 - 4.0 * x * x);
-  // q = 0 only at x = +-1, where |1 - 4 x^2| = 3
-  constexpr double kDgEdge = 1.5 * 1.7320508075688772 / 6.123233995736766e-17;
-  dg = q == 0.0 ? kDgEdge : 1.5 * sqrt(a / q);
-  const double d = dcf_value(s, m, nr, shift);
-  wt = (s < 0 || s > nr) ? 0.0 : fmin(d * dg, 0.125);
+// The dcf table, j in [0, nr]: the entries the rows gather.
+__global__ void k_dcf_table(const uint32_t* __restrict__ mb, int32_t nr, int shift, double* __restrict__ dcf) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j <= nr; j += (int64_t)gridDim.x * blockDim.x)
+    dcf[j] = dcf_value(j, mb[dcf_slot(j, nr, shift)], nr, shift);
 }
 
-// The same launch writes the dcf table (entries j = thread index + multiples
-// of the grid size).  VEC: bll 8-B and w / wt / dg 16-B aligned; each thread
-// takes 2 rows per step.  The shell index is recomputed from bll (the same
-// expression as k_shell_hist, so the same gs) instead of read back, and both
-// rows' mb loads are issued before the arithmetic.
-template <bool VEC>
-__global__ void k_row_weights(const float* __restrict__ bll, const double* __restrict__ w, int64_t K, int32_t nr,
-                              const uint32_t* __restrict__ mb, int shift, double* __restrict__ dcf,
-                              double* __restrict__ dg_out, double* __restrict__ wt) {
+// dg of one row.  With gamma = asin(x), x = w / b: sin(gamma) = x and
+// cos(gamma)^2 = 1 - x^2, so dg = 1.5 sqrt(|1 - 4 x^2| / (1 - x^2))
+// = 1.5 sqrt(|b^2 - 4 w^2| / (b^2 - w^2)): one division and one square root,
+// NaN for |w| > |b| (as asin).  b^2 is exact (b is FP32), so the denominator
+// is 0 exactly when |w| = |b|, where the asin route gives cos(fl(pi/2)) =
+// 6.1e-17, not 0: that value is kept, so dg and wt stay finite there as in
+// the formula.  The cases the squares do not cover as x = w / b does: b = 0
+// (x is infinite or NaN: NaN) and an infinite b (x = 0: 1.5, or NaN for an
+// infinite or NaN w).
+__device__ inline double row_dg(float b, double wk) {
+  constexpr double kDgEdge = 1.5 * 1.7320508075688772 / 6.123233995736766e-17;
+  const double bd = (double)b;
+  const double bb = bd * bd, ww = wk * wk;
+  const double q = bb - ww, a = fabs(bb - 4.0 * ww);
+  double dg = q == 0.0 ? kDgEdge : 1.5 * sqrt(a / q);
+  dg = bd == 0.0 ? __builtin_nan("") : dg;
+  return __builtin_isinf(bd) ? (__builtin_isfinite(wk) ? 1.5 : __builtin_nan("")) : dg;
+}
+
+// wt = fmin(dcf[gs] dg, 1/8) for gs in [0, nr], else 0; dcf[gs] gathered from the table.
+__device__ inline double row_wt(int32_t s, int32_t nr, double dg, const double* __restrict__ dcf) {
+  const bool in = s >= 0 && s <= nr;
+  const double d = dcf[in ? s : 0];
+  return in ? fmin(d * dg, 0.125) : 0.0;
+}
+
+// VEC: bll 8-B, w / wt / dg 16-B aligned; rows go in pairs (one 8-B load of
+// bll, one 16-B load of w, one 16-B store of wt and of dg), so the lanes of a
+// wavefront load and store contiguous 512-B / 1-KB spans, and each thread has
+// four pairs in flight per step.  The shell index is recomputed from bll (the
+// same expression as k_shell_hist, so the same gs) instead of read back.
+// LDS_TAB: the block first builds the dcf table in LDS from the shell counts
+// (block 0 also writes it to dcf_g) and the rows gather from there.  Rows hit
+// shells at random, so gathering from a global table would cost a cache-line
+// request per row.  Otherwise the table is in dcf_g already (k_dcf_table).
+template <bool VEC, bool LDS_TAB>
+__global__ __launch_bounds__(1024) void k_row_weights(const float* __restrict__ bll, const double* __restrict__ w,
+                                                      int64_t K, int32_t nr, const uint32_t* __restrict__ mb,
+                                                      int shift, double* __restrict__ dcf_g,
+                                                      double* __restrict__ dg_out, double* __restrict__ wt) {
+  extern __shared__ double tab_s[];
+  const double* __restrict__ dcf = dcf_g;
+  if (LDS_TAB) {
+    for (int j = threadIdx.x; j <= nr; j += blockDim.x) {
+      const double d = dcf_value(j, mb[dcf_slot(j, nr, shift)], nr, shift);
+      tab_s[j] = d;
+      if (blockIdx.x == 0) dcf_g[j] = d;
+    }
+    __syncthreads();
+    dcf = tab_s;
+  }
   const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t j = t0; j <= nr; j += stride) dcf[j] = dcf_value(j, mb[dcf_slot(j, nr, shift)], nr, shift);
   int64_t k_tail = 0;
   if (VEC) {
     const int64_t K2 = K / 2;
-    for (int64_t q = t0; q < K2; q += stride) {
-      const float2 b = reinterpret_cast<const float2*>(bll)[q];
-      const double2 wk = reinterpret_cast<const double2*>(w)[q];
-      const int32_t s0 = shell_index(b.x), s1 = shell_index(b.y);
-      const uint32_t m0 = mb[dcf_slot(s0, nr, shift)], m1 = mb[dcf_slot(s1, nr, shift)];
-      double2 dg, wo;
-      row_weight(b.x, wk.x, s0, m0, nr, shift, dg.x, wo.x);
-      row_weight(b.y, wk.y, s1, m1, nr, shift, dg.y, wo.y);
-      if (dg_out) reinterpret_cast<double2*>(dg_out)[q] = dg;
-      reinterpret_cast<double2*>(wt)[q] = wo;
+    const float2* __restrict__ b2 = reinterpret_cast<const float2*>(bll);
+    const double2* __restrict__ w2 = reinterpret_cast<const double2*>(w);
+    double2* __restrict__ t2 = reinterpret_cast<double2*>(wt);
+    double2* __restrict__ g2 = reinterpret_cast<double2*>(dg_out);
+    const auto pair = [&](const float2 b, const double2 wk, int64_t q) {
+      const double d0 = row_dg(b.x, wk.x), d1 = row_dg(b.y, wk.y);
+      t2[q] = make_double2(row_wt(shell_index(b.x), nr, d0, dcf), row_wt(shell_index(b.y), nr, d1, dcf));
+      if (dg_out) g2[q] = make_double2(d0, d1);
+    };
+    int64_t q = t0;
+    for (; q + 3 * stride < K2; q += 4 * stride) {  // four pairs in flight
+      float2 b[4];
+      double2 wk[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        b[i] = b2[q + i * stride];
+        wk[i] = w2[q + i * stride];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pair(b[i], wk[i], q + i * stride);
     }
+    for (; q < K2; q += stride) pair(b2[q], w2[q], q);
     k_tail = K2 * 2;
   }
   for (int64_t k = k_tail + t0; k < K; k += stride) {
     const float b = bll[k];
-    const int32_t sk = shell_index(b);
-    double dg, wo;
-    row_weight(b, w[k], sk, mb[dcf_slot(sk, nr, shift)], nr, shift, dg, wo);
-    if (dg_out) dg_out[k] = dg;
-    wt[k] = wo;
+    const double dgk = row_dg(b, w[k]);
+    if (dg_out) dg_out[k] = dgk;
+    wt[k] = row_wt(shell_index(b), nr, dgk, dcf);
   }
 }
+
 #pragma clang fp contract(on)
 
 }  // namespace viss
@@ -190,30 +233,55 @@ extern "C" int viss_weights(const float* bll, const double* w, int64_t K, int32_
     VISS_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     int64_t per_cu = use_lds && lds > 0 ? (int64_t)lds_cu / (int64_t)lds : 8;
     per_cu = per_cu < 2 ? 2 : (per_cu > 8 ? 8 : per_cu);
-    int64_t blocks = (K + 256 * 16 - 1) / (256 * 16);
+    // 1024-thread blocks: the per-CU block count is small (the LDS table),
+    // so large blocks keep enough loads in flight
+    const int hb = use_lds ? 1024 : 256;
+    int64_t blocks = (K + hb * 16 - 1) / (hb * 16);
     const int64_t cap = (int64_t)(ncu > 0 ? ncu : 256) * per_cu;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
     const bool vec = ((uintptr_t)bll % 16 == 0) && ((uintptr_t)gs % 16 == 0);
     if (vec)
-      k_shell_hist<true><<<(unsigned)blocks, 256, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
-                                                                         (unsigned long long*)n_out_of_range, use_lds);
+      k_shell_hist<true><<<(unsigned)blocks, hb, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
+                                                                        (unsigned long long*)n_out_of_range, use_lds);
     else
-      k_shell_hist<false><<<(unsigned)blocks, 256, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
-                                                                          (unsigned long long*)n_out_of_range, use_lds);
+      k_shell_hist<false><<<(unsigned)blocks, hb, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
+                                                                         (unsigned long long*)n_out_of_range, use_lds);
     VISS_LAUNCHED(k_shell_hist);
   }
-  {  // row weights and the dcf table (also for K = 0: the table of an empty histogram)
-    const int shift = (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0;
+  const int shift = (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0;
+  const size_t tab = ((size_t)nr + 1) * sizeof(double);
+  // the row kernel builds the table in LDS when it fits and the rows are many
+  // enough to pay for one table per block; otherwise a launch of its own writes
+  // it (also for K = 0: the table of an empty histogram)
+  const bool lds_tab = K > 0 && tab <= (size_t)lds_max && K >= 16 * ((int64_t)nr + 1);
+  if (!lds_tab) {
+    int64_t g = ((int64_t)nr + 1 + 255) / 256;
+    if (g > 4096) g = 4096;
+    k_dcf_table<<<(unsigned)g, 256, 0, s>>>(mb, nr, shift, dcf);
+    VISS_LAUNCHED(k_dcf_table);
+  }
+  if (K > 0) {  // row weights, gathering the table
     const bool vec = ((uintptr_t)bll % 8 == 0) && ((uintptr_t)w % 16 == 0) && ((uintptr_t)wt % 16 == 0) &&
                      (dg == nullptr || (uintptr_t)dg % 16 == 0);
-    int64_t g = ((vec ? K / 2 : K) + 255) / 256;
-    if (g > 65536) g = 65536;
+    // LDS path: 1024-thread blocks, as many per CU as the table copies allow
+    // (each block builds the table once); else 256-thread blocks
+    const int block = lds_tab ? 1024 : 256;
+    int lds_cu = 0;
+    VISS_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    int64_t per_cu = lds_tab ? std::max<int64_t>(1, std::min<int64_t>(2, (int64_t)lds_cu / (int64_t)tab)) : 64;
+    int64_t g = ((vec ? K / 8 : K) + block - 1) / block;  // four row pairs per thread and step
+    const int64_t cap = (int64_t)(ncu > 0 ? ncu : 256) * per_cu;
+    if (g > cap) g = cap;
     if (g < 1) g = 1;
-    if (vec)
-      k_row_weights<true><<<(unsigned)g, 256, 0, s>>>(bll, w, K, nr, mb, shift, dcf, dg, wt);
-    else
-      k_row_weights<false><<<(unsigned)g, 256, 0, s>>>(bll, w, K, nr, mb, shift, dcf, dg, wt);
+#define VISS_RW(V, L) \
+  k_row_weights<V, L><<<(unsigned)g, block, L ? tab : 0, s>>>(bll, w, K, nr, mb, shift, dcf, dg, wt)
+    if (vec) {
+      if (lds_tab) VISS_RW(true, true); else VISS_RW(true, false);
+    } else {
+      if (lds_tab) VISS_RW(false, true); else VISS_RW(false, false);
+    }
+#undef VISS_RW
     VISS_LAUNCHED(k_row_weights);
   }
   return VISS_OK;
