@@ -581,12 +581,47 @@ def batched(ops, geo, dev):
     rel = float(torch.linalg.vector_norm(V - V_group) / torch.linalg.vector_norm(V_group))
     st = batch.stats()
     log(f"[bench] batched: graph-replayed tuned batch vs per-group launches rel L2 {rel:.2e}")
+    # density-compensated imaging of every group: per-group weights (one nr for all groups) and
+    # the summed image, batched (two weight launches, one back-projection graph) against per-group
+    # viss_weights + viss_sky_backproject launches
+    bll = torch.sqrt((U * U).sum(0)).float()
+    nr = int(torch.ceil((bll.max() - 0.25) / 0.5).item()) + 1
+    yv = V_group
+    x_b = torch.zeros(sky.P, dtype=torch.float64, device=dev)
+    x_g = torch.zeros_like(x_b)
+    wt_g = torch.empty(U.shape[1], dtype=torch.float64, device=dev)
+
+    def img_group():
+        x_g.zero_()
+        for g, (u, a, b) in enumerate(groups):
+            s = slice(offs[g], offs[g + 1])
+            wt_g[s] = ops.weights(bll[s], u[2], nr, want_dg=False)["wt"]
+            sky.backproject(u, yv[s], wt_g[s], a, b, mask=True, R_moon=R, out=x_g)
+
+    def img_batch():
+        w = batch.weights(bll, nr, want_dg=False)["wt"]
+        x_b.zero_()
+        batch.backproject(yv, w, out=x_b, use_graph=True)
+
+    t_img_group = timed(img_group)
+    with torch.cuda.stream(side):
+        side.wait_stream(torch.cuda.current_stream())
+        t_img_batch = timed(img_batch)
+    torch.cuda.synchronize()
+    rel_img = float(torch.linalg.vector_norm(x_b - x_g) / torch.linalg.vector_norm(x_g))
+    log(f"[bench] batched imaging: {t_img_batch * 1e3:.1f} ms vs per-group {t_img_group * 1e3:.1f} ms, "
+        f"rel L2 {rel_img:.2e}")
+    imaging = {"batched_ms": t_img_batch * 1e3, "per_group_ms": t_img_group * 1e3,
+               "batched_pairs_per_s": visible / t_img_batch, "per_group_pairs_per_s": visible / t_img_group,
+               "speedup_batched_over_per_group": t_img_group / t_img_batch, "rel_l2_batched_vs_per_group": rel_img,
+               "nr": nr, "what": "per-group density-compensation weights and the summed back-projected image"}
     return {"groups": G, "rows": int(U.shape[1]), "npix": sky.P, "visible_pairs": visible,
             "visible_tile_fraction": st["visible_tiles"] / st["total_tiles"],
             "per_group_launches_pairs_per_s": visible / t_group, "batched_default_pairs_per_s": visible / t_default,
             "batched_tuned_graph_pairs_per_s": visible / t_graph, "speedup_batched_over_per_group": t_group / t_graph,
             "rel_l2_batched_vs_per_group": rel,
-            "autotune_s": t_tune, "work_items": st["n_items"], "work_items_untuned": st0["n_items"]}
+            "autotune_s": t_tune, "work_items": st["n_items"], "work_items_untuned": st0["n_items"],
+            "imaging": imaging}
 
 
 def hbm_passes(ops, inp, wl, dev):

@@ -20,9 +20,11 @@
  *                              hipStreamSynchronize, hipFree, hipMalloc; otherwise neither;
  *       viss_batch_create, viss_batch_autotune, viss_batch_tuning_load
  *                              allocate the work table and synchronise the stream;
- *       viss_batch_forward with use_graph: on its first call and when I, V or
- *                              the stream change, it captures and instantiates a graph
- *                              (host-side objects, no device memory, no synchronisation);
+ *       viss_batch_forward / viss_batch_backproject with use_graph: on the
+ *                              first call and when their arguments or the stream change,
+ *                              they capture and instantiate a graph (host-side objects,
+ *                              no device memory, no synchronisation);
+ *       viss_batch_backproject the first call on a batch (its row-record buffer);
  *     viss_sky_backproject_noalloc never allocates or synchronises;
  *   - a sequence without such calls (viss_sky_forward, viss_sky_backproject
  *     after its buffer is sized, viss_weights, viss_pix2ang, ...) can be
@@ -246,6 +248,37 @@ int viss_batch_forward(viss_batch* batch, const double* I, double* V, int use_gr
 int viss_batch_autotune(viss_batch* batch, const double* I, double* V_scratch, int reps, void* stream);
 int viss_batch_tuning_save(const viss_batch* batch, const char* path);
 int viss_batch_tuning_load(viss_batch* batch, const char* path, void* stream);
+/* Density-compensation weights of every group, each group counted on its own
+ * (the reference computes them per day, vissgen3d_noblockage.cu:448-507;
+ * viss_weights per group), in two launches: one shell histogram per group
+ * (LDS-privatised, one block per chunk of a group's rows) and the rows'
+ * weights, each block building its group's dcf table in LDS.  bll: [K_total]
+ * FP32 baseline lengths; w is the batch's uvw row 2.  gs, dg (nullable), wt:
+ * [K_total]; mb: [G nr] uint32, group g's counts at mb + g nr (zeroed by the
+ * caller: it accumulates); dcf: [G (nr + 1)], group g's table at
+ * dcf + g (nr + 1); n_out_of_range: total over groups (device int64,
+ * accumulated, nullable).  One nr for every group (the largest shell any row
+ * falls in, or the reference's ceil(2 bl_max / lambda) of the highest
+ * frequency): a group's weights do not depend on nr beyond its own largest
+ * shell, except with VISS_DCF_BLOCKAGE_SHIFT (dcf_nr = 0).  VISS_ERR_SIZE when
+ * a group's table, (nr + 1) doubles, exceeds one block's LDS (nr <= 20,479 on
+ * gfx950).  Launches kernels only. */
+int viss_batch_weights(viss_batch* batch, const float* bll, int32_t nr, int flags, int32_t* gs, uint32_t* mb,
+                       double* dcf, double* dg, double* wt, int64_t* n_out_of_range, void* stream);
+/* Back-projection of every group's rows, computeC per day
+ * (vissgen3d_noblockage.cu:511-527) for all groups in one prologue and one
+ * tile-kernel launch: y: complex [K_total], wt: [K_total] FP64 (e.g. from
+ * viss_batch_weights) or NULL (the adjoint).  per_group == 0: x [P] receives
+ * the sum over groups (the notebooks' multi-day total); per_group != 0: x is
+ * [G][P], group g's image at x + g P.  flags: VISS_PHASE_TRACK,
+ * VISS_HERMITIAN (as viss_sky_backproject); occultation and precision are the
+ * batch's.  The row records go to a buffer the batch owns: the first call (and
+ * one with larger records) synchronises and allocates it, later calls launch
+ * kernels only.  use_graph != 0: the launches are captured once into a
+ * hipGraph (re-captured when y, wt, x, flags, per_group or the stream change)
+ * and replayed; needs a non-default stream. */
+int viss_batch_backproject(viss_batch* batch, const double* y, const double* wt, int flags, double* x, int per_group,
+                           int use_graph, void* stream);
 /* Sizes of the batch: tiles kept over all groups, tiles before compaction,
  * work items, blocks of the batch launch (4 items per block).  Every pointer is a host pointer
  * (nullable). */

@@ -29,6 +29,7 @@ EXPORTS = (
     "viss_sky_quads", "viss_sky_forward",
     "viss_sky_backproject", "viss_sky_backproject_noalloc", "viss_sky_count_visible", "viss_sky_count_work", "viss_batch_create", "viss_batch_destroy", "viss_batch_forward",
     "viss_batch_autotune", "viss_batch_tuning_save", "viss_batch_tuning_load", "viss_batch_stats",
+    "viss_batch_weights", "viss_batch_backproject",
     "viss_image_accumulate", "viss_reorder", "viss_sky_backproject_snapshots", "viss_sky_forward_snapshots",
 )
 
@@ -86,6 +87,8 @@ def load():
         "viss_batch_tuning_save": ([_vp, ctypes.c_char_p], _int),
         "viss_batch_tuning_load": ([_vp, ctypes.c_char_p, _vp], _int),
         "viss_batch_stats": ([_vp, _vp, _vp, _vp, _vp], _int),
+        "viss_batch_weights": ([_vp, _vp, _i32, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _int),
+        "viss_batch_backproject": ([_vp, _vp, _vp, _int, _vp, _int, _int, _vp], _int),
         "viss_image_accumulate": ([_vp, _i64, _int, _vp, _vp], _int),
         "viss_reorder": ([_vp, _i64, _int, _vp, _vp], _int),
         "viss_sky_backproject_snapshots": ([_vp, _vp, _i64, _i32, ctypes.c_double, _vp, _vp, _geom_p, _int, _vp, _vp],

@@ -296,6 +296,18 @@ struct viss_batch {
   const double* g_I = nullptr;
   double* g_V = nullptr;
   hipStream_t g_stream = nullptr;
+  // back-projection and weights over the groups' rows
+  viss::RowChunk* bp_chunks = nullptr;  // row chunks within groups (back-projection)
+  int64_t n_bp_chunks = 0;
+  viss::RowChunk* w_chunks = nullptr;   // row chunks within groups, one at least per group (weights)
+  int64_t n_w_chunks = 0;
+  viss::DevBuf bp_ws;                   // row records of the back-projection (sized on first use)
+  hipGraph_t bp_graph = nullptr;
+  hipGraphExec_t bp_exec = nullptr;
+  const double *bg_y = nullptr, *bg_wt = nullptr;
+  double* bg_x = nullptr;
+  int bg_flags = -1, bg_per_group = -1;
+  hipStream_t bg_stream = nullptr;
 };
 
 namespace viss {
@@ -412,6 +424,11 @@ static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, d
 static void batch_free(viss_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->bp_exec) (void)hipGraphExecDestroy(b->bp_exec);
+  if (b->bp_graph) (void)hipGraphDestroy(b->bp_graph);
+  if (b->bp_chunks) (void)hipFree(b->bp_chunks);
+  if (b->w_chunks) (void)hipFree(b->w_chunks);
+  b->bp_ws.release();
   if (b->d_offs) (void)hipFree(b->d_offs);
   if (b->gcone) (void)hipFree(b->gcone);
   if (b->gcut) (void)hipFree(b->gcut);
@@ -421,9 +438,48 @@ static void batch_free(viss_batch* b) {
   if (b->items) (void)hipFree(b->items);
 }
 
+// Row chunks that never cross a group: back-projection chunks of at most cr
+// rows (cr doubled until the table fits a grid's y extent), and weight chunks
+// of at most 32768 rows with at least one (possibly empty) chunk per group.
+static int build_chunks(viss_batch* b, hipStream_t s) {
+  const int64_t G = b->G;
+  std::vector<RowChunk> bp, wc;
+  for (int64_t cr = 4096;; cr *= 2) {
+    bp.clear();
+    for (int64_t g = 0; g < G; ++g)
+      for (int64_t k0 = b->offs[g]; k0 < b->offs[g + 1]; k0 += cr)
+        bp.push_back(RowChunk{k0, std::min(b->offs[g + 1], k0 + cr), g, 0});
+    if ((int64_t)bp.size() <= 65535 || cr > ((int64_t)1 << 40)) break;
+  }
+  VISS_REQUIRE((int64_t)bp.size() <= 65535, "viss_batch: %zu back-projection row chunks", bp.size());
+  constexpr int64_t kWeightChunk = 32768;
+  for (int64_t g = 0; g < G; ++g) {
+    int64_t k0 = b->offs[g];
+    do {
+      const int64_t k1 = std::min(b->offs[g + 1], k0 + kWeightChunk);
+      wc.push_back(RowChunk{k0, k1, g, k0 == b->offs[g] ? 1 : 0});
+      k0 = k1;
+    } while (k0 < b->offs[g + 1]);
+  }
+  b->n_bp_chunks = (int64_t)bp.size();
+  b->n_w_chunks = (int64_t)wc.size();
+  if (!bp.empty()) {
+    VISS_HIP(hipMalloc((void**)&b->bp_chunks, bp.size() * sizeof(RowChunk)));
+    VISS_HIP(hipMemcpyAsync(b->bp_chunks, bp.data(), bp.size() * sizeof(RowChunk), hipMemcpyHostToDevice, s));
+  }
+  VISS_HIP(hipMalloc((void**)&b->w_chunks, wc.size() * sizeof(RowChunk)));
+  VISS_HIP(hipMemcpyAsync(b->w_chunks, wc.data(), wc.size() * sizeof(RowChunk), hipMemcpyHostToDevice, s));
+  VISS_HIP(hipStreamSynchronize(s));  // the host tables go out of scope
+  return VISS_OK;
+}
+
 static int batch_build(viss_batch* b, hipStream_t s) {
   viss_sky* sky = b->sky;
   const int64_t G = b->G;
+  {
+    const int rc = build_chunks(b, s);
+    if (rc != VISS_OK) return rc;
+  }
   VISS_HIP(hipMalloc((void**)&b->d_offs, sizeof(int64_t) * (G + 1)));
   VISS_HIP(hipMalloc((void**)&b->gcone, sizeof(float4) * G));
   VISS_HIP(hipMalloc((void**)&b->gcut, sizeof(float) * G));
@@ -656,6 +712,71 @@ int viss_batch_tuning_load(viss_batch* b, const char* path, void* stream) {
   std::vector<WorkItem> v;
   build_items(b, groups, v, b->tune, 0);
   return upload_items(b, v, as_stream(stream));
+}
+
+int viss_batch_weights(viss_batch* b, const float* bll, int32_t nr, int flags, int32_t* gs, uint32_t* mb,
+                       double* dcf, double* dg, double* wt, int64_t* n_out_of_range, void* stream) {
+  VISS_REQUIRE(b != nullptr, "viss_batch_weights: batch is NULL");
+  int rc = check_device(b->sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE(nr >= 1 && nr <= (1 << 26), "viss_batch_weights: nr %d outside [1, 2^26]", nr);
+  VISS_REQUIRE(mb && dcf, "viss_batch_weights: mb/dcf must not be NULL");
+  VISS_REQUIRE(b->Ktot == 0 || (bll && gs && wt), "viss_batch_weights: bll/gs/wt must not be NULL");
+  VISS_REQUIRE((flags & ~(VISS_DCF_BLOCKAGE_SHIFT)) == 0, "viss_batch_weights: unsupported flags 0x%x", flags);
+  return batch_weights(bll, b->uvw + 2 * b->Ktot, b->w_chunks, b->n_w_chunks, nr, flags, gs, mb, dcf, dg, wt,
+                       n_out_of_range, as_stream(stream));
+}
+
+// Prologue and tile kernels of the back-projection over every group's rows,
+// on the batch's row chunks.
+static int launch_batch_bp(viss_batch* b, const double* y, const double* wt, int flags, double* x, int per_group,
+                           hipStream_t s) {
+  const int f = (b->flags & (VISS_MASK | VISS_PREC_FP64)) | flags;
+  const viss_mask_geometry geom{b->R, b->orb};
+  return sky_backproject_ws(b->sky, b->uvw, b->xa, b->xb, y, wt, b->Ktot, &geom, f, x, (char*)b->bp_ws.p, s,
+                            b->bp_chunks, b->n_bp_chunks, per_group ? b->sky->P : 0);
+}
+
+int viss_batch_backproject(viss_batch* b, const double* y, const double* wt, int flags, double* x, int per_group,
+                           int use_graph, void* stream) {
+  VISS_REQUIRE(b != nullptr && y != nullptr && x != nullptr, "viss_batch_backproject: NULL argument");
+  int rc = check_device(b->sky);
+  if (rc != VISS_OK) return rc;
+  VISS_REQUIRE((flags & ~(VISS_PHASE_TRACK | VISS_HERMITIAN)) == 0, "viss_batch_backproject: unsupported flags 0x%x",
+               flags);
+  if (b->Ktot == 0) return VISS_OK;
+  hipStream_t s = as_stream(stream);
+  VISS_REQUIRE(!use_graph || s != nullptr,
+               "viss_batch_backproject: graph mode needs a non-default stream (capture is not allowed on stream 0)");
+  rc = b->bp_ws.reserve(bp_workspace_bytes(b->Ktot, (b->flags & (VISS_MASK | VISS_PREC_FP64)) | flags), s);
+  if (rc != VISS_OK) return rc;
+  if (!use_graph) return launch_batch_bp(b, y, wt, flags, x, per_group, s);
+  if (!b->bp_exec || b->bg_y != y || b->bg_wt != wt || b->bg_x != x || b->bg_flags != flags ||
+      b->bg_per_group != (per_group ? 1 : 0) || b->bg_stream != s) {
+    if (b->bp_exec) (void)hipGraphExecDestroy(b->bp_exec);
+    if (b->bp_graph) (void)hipGraphDestroy(b->bp_graph);
+    b->bp_exec = nullptr;
+    b->bp_graph = nullptr;
+    VISS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = launch_batch_bp(b, y, wt, flags, x, per_group, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != VISS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    b->bp_graph = g;
+    VISS_HIP(hipGraphInstantiate(&b->bp_exec, b->bp_graph, nullptr, nullptr, 0));
+    b->bg_y = y;
+    b->bg_wt = wt;
+    b->bg_x = x;
+    b->bg_flags = flags;
+    b->bg_per_group = per_group ? 1 : 0;
+    b->bg_stream = s;
+  }
+  VISS_HIP(hipGraphLaunch(b->bp_exec, s));
+  return VISS_OK;
 }
 
 int viss_batch_stats(const viss_batch* b, int64_t* visible_tiles, int64_t* total_tiles, int64_t* n_items,

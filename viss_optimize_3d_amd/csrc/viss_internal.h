@@ -188,7 +188,27 @@ struct viss_sky {
 };
 
 namespace viss {
+// A chunk of back-projection rows [k0, k1) of group g (see sky_backproject_ws).
+struct RowChunk {
+  int64_t k0, k1, g, pad;
+};
+static_assert(sizeof(RowChunk) == 32, "RowChunk is 32 B");
+
 int sky_build(viss_sky* sky, const double* lmn, int64_t P, hipStream_t s);
+// Bytes of the row records the back-projection packs for K rows with these flags.
+size_t bp_workspace_bytes(int64_t K, int flags);
+// The back-projection over a workspace of bp_workspace_bytes(K, flags) bytes at
+// ws (256-B aligned), arguments already checked.  chunks (device, nchunks
+// entries) replaces the fixed row chunks: block row c takes rows
+// [chunks[c].k0, chunks[c].k1) and adds into x + chunks[c].g * img_stride.
+// Per-group weights of a batch over weight chunks wch (device; pad != 0 on a
+// group's first chunk): histograms mb [G nr], tables dcf [G (nr + 1)].
+int batch_weights(const float* bll, const double* w, const RowChunk* wch, int64_t nwch, int32_t nr, int flags,
+                  int32_t* gs, uint32_t* mb, double* dcf, double* dg, double* wt, int64_t* n_out_of_range,
+                  hipStream_t s);
+int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb, const double* y,
+                       const double* wt, int64_t K, const viss_mask_geometry* geom, int flags, double* x, char* ws,
+                       hipStream_t s, const RowChunk* chunks, int64_t nchunks, int64_t img_stride);
 void sky_free(viss_sky* sky);
 int sky_set_brightness(viss_sky* sky, const double* I, bool want64, hipStream_t s);
 int check_device(const viss_sky* sky);

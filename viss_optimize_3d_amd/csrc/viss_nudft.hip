@@ -106,6 +106,28 @@ __global__ __launch_bounds__(256) void k_forward_quads(const QuadPack* __restric
 }
 
 // ------------------------------------------------------------ back-projection
+// Row chunks of the back-projection kernels (grid.y): by default chunk c holds
+// rows [c kpc, (c + 1) kpc); a batch passes a table whose chunks never cross a
+// group, with the group's image at x + g img_stride (0: one summed image).
+struct RowChunks {
+  const RowChunk* tab;
+  int64_t img_stride;
+};
+
+__device__ __forceinline__ void chunk_rows(const RowChunks& rc, int64_t K, int64_t kpc, int64_t& k0, int64_t& k1,
+                                           int64_t& xoff) {
+  if (rc.tab) {
+    const RowChunk c = rc.tab[blockIdx.y];  // wave-uniform
+    k0 = c.k0;
+    k1 = c.k1;
+    xoff = c.g * rc.img_stride;
+  } else {
+    k0 = (int64_t)blockIdx.y * kpc;
+    k1 = min(K, k0 + kpc);
+    xoff = 0;
+  }
+}
+
 // Prologue: y'_k = wt_k * y_k [* exp(-2 pi i w_k)], packed with u, v, w.
 __global__ void k_bp_prologue(const double* __restrict__ uvw, const double* __restrict__ xyza,
                               const double* __restrict__ xyzb, const double* __restrict__ y,
@@ -157,7 +179,7 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
                                                      const VisHot64* __restrict__ hot64,
                                                      const VisMask* __restrict__ vm, const VisCls* __restrict__ vc,
                                                      int64_t K, int64_t k_per_chunk, double herm,
-                                                     double* __restrict__ x) {
+                                                     double* __restrict__ x, const RowChunks rc) {
   constexpr int PPL = kBpPixPerLane;
   const int lane = threadIdx.x & 63;
   const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
@@ -174,8 +196,9 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
   }
   float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
   if (MASK) cn = bcone[tile];
-  const int64_t k0 = (int64_t)blockIdx.y * k_per_chunk;
-  const int64_t k1 = min(K, k0 + k_per_chunk);
+  int64_t k0, k1, xoff;
+  chunk_rows(rc, K, k_per_chunk, k0, k1, xoff);
+  double* __restrict__ xo = x + xoff;
   double acc[PPL];
 #pragma unroll
   for (int i = 0; i < PPL; ++i) acc[i] = 0.0;
@@ -247,7 +270,7 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
   }
 #pragma unroll
   for (int i = 0; i < PPL; ++i)
-    if (idx[i] >= 0) unsafeAtomicAdd(&x[idx[i]], herm * acc[i]);
+    if (idx[i] >= 0) unsafeAtomicAdd(&xo[idx[i]], herm * acc[i]);
 }
 
 // ------------------------------------------------------------ back-projection, antipodal pairs
@@ -264,7 +287,8 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
                                                            const VisHot* __restrict__ hot,
                                                            const VisMask* __restrict__ vm,
                                                            const VisCls* __restrict__ vc, int64_t K,
-                                                           int64_t k_per_chunk, double herm, double* __restrict__ x) {
+                                                           int64_t k_per_chunk, double herm, double* __restrict__ x,
+                                                           const RowChunks rc) {
   const int lane = threadIdx.x & 63;
   const int64_t tile = (int64_t)blockIdx.x * 4 + wave_in_block();
   if (tile >= ntile) return;  // wave-uniform
@@ -273,8 +297,9 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
   const double l = q.l, m = q.m, n = q.n;
   float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
   if (MASK) cn = pcone[tile];
-  const int64_t k0 = (int64_t)blockIdx.y * k_per_chunk;
-  const int64_t k1 = min(K, k0 + k_per_chunk);
+  int64_t k0, k1, xoff;
+  chunk_rows(rc, K, k_per_chunk, k0, k1, xoff);
+  double* __restrict__ xo = x + xoff;
   double acc_a = 0.0, acc_b = 0.0;
   for (int64_t kb = k0; kb < k1; kb += 64) {
     const int nk = (int)min((int64_t)64, k1 - kb);
@@ -337,8 +362,8 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
     acc_a += (double)sa;
     acc_b += (double)sb;
   }
-  if (ab.x >= 0) unsafeAtomicAdd(&x[ab.x], herm * acc_a);
-  if (ab.y >= 0) unsafeAtomicAdd(&x[ab.y], herm * acc_b);
+  if (ab.x >= 0) unsafeAtomicAdd(&xo[ab.x], herm * acc_a);
+  if (ab.y >= 0) unsafeAtomicAdd(&xo[ab.y], herm * acc_b);
 }
 
 // ------------------------------------------------------------ back-projection, equatorial-mirror quads
@@ -409,7 +434,7 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     const QuadPack* __restrict__ qp, const int4* __restrict__ qi, const double* __restrict__ qz,
     const int4* __restrict__ patch, const float4* __restrict__ pcone, int64_t npatch, const VisHot* __restrict__ hot,
     const VisMask* __restrict__ vm, const VisCls* __restrict__ vc, int64_t K, int64_t k_per_chunk, double herm,
-    double* __restrict__ x) {
+    double* __restrict__ x, const RowChunks rc) {
   __shared__ float4 rot_s[4][4][64];  // [wavefront][ring group][row]
   const int lane = threadIdx.x & 63;
   const int wv = wave_in_block();
@@ -434,8 +459,9 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     cnA = pcone[2 * p];
     cnB = pcone[2 * p + 1];
   }
-  const int64_t k0 = (int64_t)blockIdx.y * k_per_chunk;
-  const int64_t k1 = min(K, k0 + k_per_chunk);
+  int64_t k0, k1, xoff;
+  chunk_rows(rc, K, k_per_chunk, k0, k1, xoff);
+  double* __restrict__ xo = x + xoff;
   double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
   float4* __restrict__ rot_w = &rot_s[wv][0][0];
   QuadBpAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // FP32 sums of up to 4 steps (256 rows)
@@ -503,10 +529,10 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   if (nstep > 0) flush();
-  if (idx.x >= 0) unsafeAtomicAdd(&x[idx.x], herm * sa);
-  if (idx.y >= 0) unsafeAtomicAdd(&x[idx.y], herm * sb);
-  if (idx.z >= 0) unsafeAtomicAdd(&x[idx.z], herm * sc);
-  if (idx.w >= 0) unsafeAtomicAdd(&x[idx.w], herm * sd);
+  if (idx.x >= 0) unsafeAtomicAdd(&xo[idx.x], herm * sa);
+  if (idx.y >= 0) unsafeAtomicAdd(&xo[idx.y], herm * sb);
+  if (idx.z >= 0) unsafeAtomicAdd(&xo[idx.z], herm * sc);
+  if (idx.w >= 0) unsafeAtomicAdd(&xo[idx.w], herm * sd);
 }
 
 // ------------------------------------------------------------ visible-pair count
@@ -712,7 +738,7 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
 }
 
 // Bytes of row records the back-projection packs for K rows (k_bp_prologue).
-static size_t bp_workspace_bytes(int64_t K, int flags) {
+size_t bp_workspace_bytes(int64_t K, int flags) {
   const size_t hot = (flags & VISS_PREC_FP64) ? sizeof(VisHot64) : sizeof(VisHot);
   const size_t msk = (flags & VISS_MASK) ? sizeof(VisMask) + sizeof(VisCls) : 0;
   return (size_t)K * (hot + msk) + 256;
@@ -731,9 +757,10 @@ static int check_backproject_args(const double* uvw, const double* xyza, const d
 // The back-projection over a workspace of at least bp_workspace_bytes(K,
 // flags) bytes at `ws` (256-B aligned): the prologue packs the row records
 // there, then the tile kernels read them.  Arguments already checked.
-static int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb,
-                              const double* y, const double* wt, int64_t K, const viss_mask_geometry* geom,
-                              int flags, double* x, char* ws, hipStream_t s) {
+// chunks / nchunks: the batch's row-chunk table (see RowChunk), or none.
+int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xyza, const double* xyzb, const double* y,
+                       const double* wt, int64_t K, const viss_mask_geometry* geom, int flags, double* x, char* ws,
+                       hipStream_t s, const RowChunk* chunks, int64_t nchunks, int64_t img_stride) {
   if (K == 0) return VISS_OK;
   const bool mask = flags & VISS_MASK;
   const bool f64 = flags & VISS_PREC_FP64;
@@ -763,33 +790,36 @@ static int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xy
   }
   kpc = cdiv(kpc, 64) * 64;
   if (cdiv(K, kpc) > 65535) kpc = cdiv(cdiv(K, 65535), 64) * 64;
-  dim3 grid((unsigned)bx, (unsigned)cdiv(K, kpc));
+  const RowChunks rc{chunks, img_stride};
+  VISS_REQUIRE(!chunks || (nchunks >= 1 && nchunks <= 65535), "backproject: %lld row chunks", (long long)nchunks);
+  dim3 grid((unsigned)bx, (unsigned)(chunks ? nchunks : cdiv(K, kpc)));
   const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
 #define VISS_BP(M, F) \
-  k_backproject<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, kpc, herm, x)
+  k_backproject<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, kpc, herm, x, rc)
   if (pairs && sky->use_quads) {
     // quad patches, then the pairs and singles outside the quads
     dim3 qgrid((unsigned)cdiv(sky->nqpatch, 4), grid.y);
     if (mask)
       k_backproject_quads<true><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,
-                                                      sky->nqpatch, hot, vm, vc, K, kpc, herm, x);
+                                                      sky->nqpatch, hot, vm, vc, K, kpc, herm, x, rc);
     else
       k_backproject_quads<false><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,
-                                                       sky->nqpatch, hot, vm, vc, K, kpc, herm, x);
+                                                       sky->nqpatch, hot, vm, vc, K, kpc, herm, x, rc);
     VISS_LAUNCHED(k_backproject_quads);
     if (sky->nrtile > 0) {
       // few tiles (the equator ring of HEALPix): shorter row chunks, so that
       // about 16k wavefronts share the work rather than a few long ones
+      // (a batch's chunk table keeps its own chunks)
       int64_t rpc = cdiv(cdiv(K * sky->nrtile, 16384), 64) * 64;
       rpc = std::min(std::max<int64_t>(rpc, 64), kpc);
       if (cdiv(K, rpc) > 65535) rpc = cdiv(cdiv(K, 65535), 64) * 64;
-      dim3 rgrid((unsigned)cdiv(sky->nrtile, 4), (unsigned)cdiv(K, rpc));
+      dim3 rgrid((unsigned)cdiv(sky->nrtile, 4), (unsigned)(chunks ? nchunks : cdiv(K, rpc)));
       if (mask)
         k_backproject_pairs<true><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm, vc,
-                                                        K, rpc, herm, x);
+                                                        K, rpc, herm, x, rc);
       else
         k_backproject_pairs<false><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm,
-                                                         vc, K, rpc, herm, x);
+                                                         vc, K, rpc, herm, x, rc);
       VISS_LAUNCHED(k_backproject_pairs);
     }
     return VISS_OK;
@@ -797,10 +827,10 @@ static int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xy
   if (pairs) {
     if (mask)
       k_backproject_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc, K,
-                                                     kpc, herm, x);
+                                                     kpc, herm, x, rc);
     else
       k_backproject_pairs<false><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc,
-                                                      K, kpc, herm, x);
+                                                      K, kpc, herm, x, rc);
   } else if (mask) {
     if (f64) VISS_BP(true, true); else VISS_BP(true, false);
   } else {
@@ -819,7 +849,7 @@ static int sky_backproject_impl(viss_sky* sky, const double* uvw, const double* 
   if (rc != VISS_OK || K == 0) return rc;
   rc = sky->ws.reserve(bp_workspace_bytes(K, flags), s);
   if (rc != VISS_OK) return rc;
-  return sky_backproject_ws(sky, uvw, xyza, xyzb, y, wt, K, geom, flags, x, (char*)sky->ws.p, s);
+  return sky_backproject_ws(sky, uvw, xyza, xyzb, y, wt, K, geom, flags, x, (char*)sky->ws.p, s, nullptr, 0, 0);
 }
 
 // ------------------------------------------------------------ allocation-free back-projection
@@ -892,7 +922,7 @@ int viss_sky_backproject_noalloc(viss_sky* sky, const double* uvw, const double*
   char* ws = nullptr;
   rc = scratch_ptr(&ws);
   if (rc != VISS_OK) return rc;
-  return sky_backproject_ws(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, ws, as_stream(stream));
+  return sky_backproject_ws(sky, uvw, xyz_a, xyz_b, y, wt, K, geom, flags, x, ws, as_stream(stream), nullptr, 0, 0);
 }
 
 int viss_forward(const double* lmn, const double* I, int64_t P, const double* uvw, const double* xyz_a,

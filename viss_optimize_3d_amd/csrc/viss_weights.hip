@@ -205,7 +205,86 @@ __global__ __launch_bounds__(1024) void k_row_weights(const float* __restrict__ 
   }
 }
 
+// ---- per-group weights of a batch: one block per chunk of one group's rows
+// (RowChunk: rows [k0, k1) of group g; pad != 0 marks the group's first chunk).
+// Each group has its own histogram mb + g nr and table dcf + g (nr + 1).
+__global__ __launch_bounds__(1024) void k_batch_shell_hist(const float* __restrict__ bll,
+                                                           const RowChunk* __restrict__ ch, int32_t nr,
+                                                           int32_t* __restrict__ gs, uint32_t* __restrict__ mb,
+                                                           unsigned long long* __restrict__ n_bad) {
+  extern __shared__ uint32_t hist[];
+  const RowChunk c = ch[blockIdx.x];
+  for (int j = threadIdx.x; j < nr; j += blockDim.x) hist[j] = 0u;
+  __syncthreads();
+  unsigned long long bad = 0;
+  for (int64_t k = c.k0 + threadIdx.x; k < c.k1; k += blockDim.x) {
+    const int32_t g = shell_index(bll[k]);
+    gs[k] = g;
+    count_shell(g, nr, hist, nullptr, 1, bad);
+  }
+  if (n_bad) {
+    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(n_bad, bad);
+  }
+  __syncthreads();
+  uint32_t* __restrict__ mbg = mb + c.g * (int64_t)nr;
+  for (int j = threadIdx.x; j < nr; j += blockDim.x) {
+    const uint32_t cnt = hist[j];
+    if (cnt) atomicAdd(&mbg[j], cnt);
+  }
+}
+
+// The group's dcf table in LDS (its first chunk's block writes it out), then
+// the chunk's rows as k_row_weights.
+__global__ __launch_bounds__(1024) void k_batch_row_weights(const float* __restrict__ bll,
+                                                            const double* __restrict__ w,
+                                                            const RowChunk* __restrict__ ch, int32_t nr,
+                                                            const uint32_t* __restrict__ mb, int shift,
+                                                            double* __restrict__ dcf, double* __restrict__ dg_out,
+                                                            double* __restrict__ wt) {
+  extern __shared__ double tab_s[];
+  const RowChunk c = ch[blockIdx.x];
+  const uint32_t* __restrict__ mbg = mb + c.g * (int64_t)nr;
+  double* __restrict__ dcfg = dcf + c.g * ((int64_t)nr + 1);
+  for (int j = threadIdx.x; j <= nr; j += blockDim.x) {
+    const double d = dcf_value(j, mbg[dcf_slot(j, nr, shift)], nr, shift);
+    tab_s[j] = d;
+    if (c.pad) dcfg[j] = d;
+  }
+  __syncthreads();
+  for (int64_t k = c.k0 + threadIdx.x; k < c.k1; k += blockDim.x) {
+    const float b = bll[k];
+    const double dgk = row_dg(b, w[k]);
+    if (dg_out) dg_out[k] = dgk;
+    wt[k] = row_wt(shell_index(b), nr, dgk, tab_s);
+  }
+}
+
 #pragma clang fp contract(on)
+
+// Per-group weights of a batch (viss_batch_weights): two launches over the
+// weight chunks, arguments checked by the caller.
+int batch_weights(const float* bll, const double* w, const RowChunk* wch, int64_t nwch, int32_t nr, int flags,
+                  int32_t* gs, uint32_t* mb, double* dcf, double* dg, double* wt, int64_t* n_out_of_range,
+                  hipStream_t s) {
+  int dev = 0, lds_max = 0;
+  VISS_HIP(hipGetDevice(&dev));
+  VISS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t tab = ((size_t)nr + 1) * sizeof(double);
+  if (tab > (size_t)lds_max) {
+    set_error("viss_batch_weights: %d shells need a %zu-B table per group, above the %d B of LDS of a block", nr,
+              tab, lds_max);
+    return VISS_ERR_SIZE;
+  }
+  VISS_REQUIRE(nwch >= 1 && nwch <= INT32_MAX, "viss_batch_weights: %lld weight chunks", (long long)nwch);
+  k_batch_shell_hist<<<(unsigned)nwch, 1024, (size_t)nr * sizeof(uint32_t), s>>>(
+      bll, wch, nr, gs, mb, (unsigned long long*)n_out_of_range);
+  VISS_LAUNCHED(k_batch_shell_hist);
+  const int shift = (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0;
+  k_batch_row_weights<<<(unsigned)nwch, 1024, tab, s>>>(bll, w, wch, nr, mb, shift, dcf, dg, wt);
+  VISS_LAUNCHED(k_batch_row_weights);
+  return VISS_OK;
+}
 
 }  // namespace viss
 

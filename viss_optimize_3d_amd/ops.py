@@ -322,6 +322,45 @@ class Batch:
         scratch = torch.zeros(self.K, dtype=torch.complex128, device=self.sky.device)
         check("viss_batch_autotune", L.viss_batch_autotune(self._h, _p(I), _p(scratch), int(reps), _stream()))
 
+    def weights(self, bll: torch.Tensor, nr: int, blockage_shift: bool = False, want_dg: bool = True):
+        """Every group's density-compensation weights (viss_batch_weights): dict with gs, wt, dg
+        ([K_total]), mb ([G, nr]), dcf ([G, nr + 1]) and n_out_of_range (total)."""
+        L = _lib.load()
+        bll = _dev(bll, torch.float32)
+        if bll.numel() != self.K:
+            raise ValueError("bll must have K_total entries")
+        dev = self.sky.device
+        gs = torch.empty(self.K, dtype=torch.int32, device=dev)
+        mb = torch.zeros(self.G, nr, dtype=torch.int32, device=dev)
+        dcf = torch.empty(self.G, nr + 1, dtype=torch.float64, device=dev)
+        dg = torch.empty(self.K, dtype=torch.float64, device=dev) if want_dg else None
+        wt = torch.empty(self.K, dtype=torch.float64, device=dev)
+        bad = torch.zeros(1, dtype=torch.int64, device=dev)
+        check("viss_batch_weights", L.viss_batch_weights(self._h, _p(bll), int(nr),
+                                                         _lib.DCF_BLOCKAGE_SHIFT if blockage_shift else 0, _p(gs),
+                                                         _p(mb), _p(dcf), _p(dg), _p(wt), _p(bad), _stream()))
+        return {"gs": gs, "mb": mb, "dcf": dcf, "dg": dg, "wt": wt, "n_out_of_range": bad}
+
+    def backproject(self, y: torch.Tensor, wt: Optional[torch.Tensor] = None, *, per_group: bool = False,
+                    phase_track: bool = False, hermitian: bool = False, out: Optional[torch.Tensor] = None,
+                    use_graph: bool = False) -> torch.Tensor:
+        """Back-projection of every group's rows (viss_batch_backproject): the summed image [P],
+        or one image per group [G, P] with per_group."""
+        L = _lib.load()
+        y = _dev(y, torch.complex128)
+        if y.numel() != self.K:
+            raise ValueError("y must have K_total entries")
+        w = None if wt is None else _dev(wt, torch.float64)
+        shape = (self.G, self.sky.P) if per_group else (self.sky.P,)
+        if out is None:
+            out = torch.zeros(shape, dtype=torch.float64, device=self.sky.device)
+        elif out.dtype != torch.float64 or tuple(out.shape) != shape or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous float64 tensor of shape {shape}")
+        check("viss_batch_backproject", L.viss_batch_backproject(
+            self._h, _p(y), _p(w), _flags(False, phase_track, False, hermitian), _p(out), 1 if per_group else 0,
+            1 if use_graph else 0, _stream()))
+        return out
+
     def save_tuning(self, path: str):
         check("viss_batch_tuning_save", _lib.load().viss_batch_tuning_save(self._h, path.encode()))
 
