@@ -356,6 +356,38 @@ def main():
                            "rel_l2_adjoint": rel_x, "max_err_forward_rel_to_max": max_v,
                            "max_err_adjoint_rel_to_max": max_x}
         log(f"[bench] fp64 GPU: {w64 * 1e3:.1f} ms/step; mixed/fp64 {value / v64:.2f}x; rel L2 fwd {rel_v:.2e} adj {rel_x:.2e}")
+    if not args.no_fp64:
+        # the precision-only comparison: the mixed per-pixel kernels (the all-FP64 path's layout) on the
+        # same directions, each moved by ~1e-9 so that no pixel pairs with its antipode
+        rng_p = np.random.default_rng(99)
+        lmn_p = lmn + torch.from_numpy(rng_p.normal(0.0, 1e-9, lmn.shape)).to(dev)
+        lmn_p = lmn_p / torch.linalg.vector_norm(lmn_p, dim=0, keepdim=True)
+        sky_p = ops.Sky(lmn_p)
+        assert not sky_p.pairs()["uses_pairs"]
+        vis_p = int(sky_p.count_visible(xa, xb, R_moon=R).item())
+        Vp = torch.zeros_like(V)
+        xp = torch.zeros_like(x)
+
+        def step_p():
+            Vp.zero_()
+            xp.zero_()
+            sky_p.forward(I, uvw, xa, xb, mask=True, R_moon=R, out=Vp)
+            sky_p.backproject(uvw, Vp, None, xa, xb, mask=True, R_moon=R, out=xp)
+
+        step_p()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step_p()
+        torch.cuda.synchronize()
+        wp = time.perf_counter() - t0
+        out["fp64_gpu"]["same_layout"] = {
+            "mixed_per_pixel_ms_per_step": wp * 1e3, "fp64_per_pixel_ms_per_step": out["fp64_gpu"]["ms_per_step"],
+            "speedup_mixed_over_fp64": out["fp64_gpu"]["ms_per_step"] / (wp * 1e3), "visible_pairs": vis_p,
+            "note": "both with the per-pixel kernels (one sin/cos per visibility-pixel pair): the precision-only "
+                    "speedup; speedup_mixed_over_fp64 above also has the quad kernels' shared sin/cos"}
+        log(f"[bench] same layout: mixed per-pixel {wp * 1e3:.1f} ms/step, all-FP64 {w64 * 1e3:.1f} ms/step "
+            f"({w64 / wp:.2f}x)")
+        del sky_p, Vp, xp, lmn_p
     V64 = V.clone() if not args.no_fp64 else None  # the all-FP64 results, for the snapshot paths' errors
     x64 = x.clone() if not args.no_fp64 else None
 

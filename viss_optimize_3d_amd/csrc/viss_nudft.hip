@@ -114,9 +114,13 @@ struct RowChunks {
   int64_t img_stride;
 };
 
+// CHUNKED is a template parameter of the kernels, so that the per-row entry
+// points' kernels carry none of the table's code or registers (four more
+// SGPRs cost the quad back-projection one wavefront per SIMD).
+template <bool CHUNKED>
 __device__ __forceinline__ void chunk_rows(const RowChunks& rc, int64_t K, int64_t kpc, int64_t& k0, int64_t& k1,
                                            int64_t& xoff) {
-  if (rc.tab) {
+  if (CHUNKED) {
     const RowChunk c = rc.tab[blockIdx.y];  // wave-uniform
     k0 = c.k0;
     k1 = c.k1;
@@ -173,7 +177,7 @@ __global__ void k_bp_prologue(const double* __restrict__ uvw, const double* __re
 
 // Grid: x = ceil(nbtile / 4) blocks of 4 wavefronts (one kBpTile-pixel tile per
 // wavefront), y = visibility chunks of `k_per_chunk` rows.
-template <bool MASK, bool F64>
+template <bool MASK, bool F64, bool CHUNKED>
 __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__ pix, const float4* __restrict__ bcone,
                                                      int64_t nbtile, const VisHot* __restrict__ hot,
                                                      const VisHot64* __restrict__ hot64,
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
   float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
   if (MASK) cn = bcone[tile];
   int64_t k0, k1, xoff;
-  chunk_rows(rc, K, k_per_chunk, k0, k1, xoff);
+  chunk_rows<CHUNKED>(rc, K, k_per_chunk, k0, k1, xoff);
   double* __restrict__ xo = x + xoff;
   double acc[PPL];
 #pragma unroll
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(256) void k_backproject(const PixPack* __restrict__
 // classified 64 at a time (lane j tests row j) for both sides of the tile and
 // run in four loops: both sides seen whole, only a, only b, and the rest with
 // the exact per-pixel test.
-template <bool MASK>
+template <bool MASK, bool CHUNKED>
 __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __restrict__ pp,
                                                            const int2* __restrict__ pidx,
                                                            const float4* __restrict__ pcone, int64_t ntile,
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(256) void k_backproject_pairs(const PairPack* __res
   float4 cn = make_float4(0.f, 0.f, 1.f, 2.f);
   if (MASK) cn = pcone[tile];
   int64_t k0, k1, xoff;
-  chunk_rows(rc, K, k_per_chunk, k0, k1, xoff);
+  chunk_rows<CHUNKED>(rc, K, k_per_chunk, k0, k1, xoff);
   double* __restrict__ xo = x + xoff;
   double acc_a = 0.0, acc_b = 0.0;
   for (int64_t kb = k0; kb < k1; kb += 64) {
@@ -429,7 +433,7 @@ __device__ __forceinline__ void quad_bp_rows(uint64_t msk, const VisHot* __restr
 // lane j loads row j, writes its y rotated by the four rings' factors to the
 // wavefront's LDS slice and classifies the row against the patch's four
 // sides; the rows then run in one loop per (ring z, ring -z) mode pair.
-template <bool MASK>
+template <bool MASK, bool CHUNKED>
 __global__ __launch_bounds__(256) void k_backproject_quads(
     const QuadPack* __restrict__ qp, const int4* __restrict__ qi, const double* __restrict__ qz,
     const int4* __restrict__ patch, const float4* __restrict__ pcone, int64_t npatch, const VisHot* __restrict__ hot,
@@ -460,7 +464,7 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     cnB = pcone[2 * p + 1];
   }
   int64_t k0, k1, xoff;
-  chunk_rows(rc, K, k_per_chunk, k0, k1, xoff);
+  chunk_rows<CHUNKED>(rc, K, k_per_chunk, k0, k1, xoff);
   double* __restrict__ xo = x + xoff;
   double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
   float4* __restrict__ rot_w = &rot_s[wv][0][0];
@@ -794,17 +798,28 @@ int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xyza, con
   VISS_REQUIRE(!chunks || (nchunks >= 1 && nchunks <= 65535), "backproject: %lld row chunks", (long long)nchunks);
   dim3 grid((unsigned)bx, (unsigned)(chunks ? nchunks : cdiv(K, kpc)));
   const double herm = (flags & VISS_HERMITIAN) ? 2.0 : 1.0;
-#define VISS_BP(M, F) \
-  k_backproject<M, F><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, kpc, herm, x, rc)
+#define VISS_BP(M, F)                                                                                        \
+  do {                                                                                                       \
+    if (chunks)                                                                                              \
+      k_backproject<M, F, true><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, \
+                                                     kpc, herm, x, rc);                                      \
+    else                                                                                                     \
+      k_backproject<M, F, false><<<grid, 256, 0, s>>>(sky->pix, sky->bcone, sky->nbtile, hot, hot64, vm, vc, K, \
+                                                      kpc, herm, x, rc);                                     \
+  } while (0)
+#define VISS_BPQ(M, C)                                                                                           \
+  k_backproject_quads<M, C><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,      \
+                                                  sky->nqpatch, hot, vm, vc, K, kpc, herm, x, rc)
+#define VISS_BPP(G, PACK, IDX, CONE, NT, KPC, M, C) \
+  k_backproject_pairs<M, C><<<G, 256, 0, s>>>(PACK, IDX, CONE, NT, hot, vm, vc, K, KPC, herm, x, rc)
   if (pairs && sky->use_quads) {
     // quad patches, then the pairs and singles outside the quads
     dim3 qgrid((unsigned)cdiv(sky->nqpatch, 4), grid.y);
-    if (mask)
-      k_backproject_quads<true><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,
-                                                      sky->nqpatch, hot, vm, vc, K, kpc, herm, x, rc);
-    else
-      k_backproject_quads<false><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, sky->qpatch, sky->qpcone,
-                                                       sky->nqpatch, hot, vm, vc, K, kpc, herm, x, rc);
+    if (mask) {
+      if (chunks) VISS_BPQ(true, true); else VISS_BPQ(true, false);
+    } else {
+      if (chunks) VISS_BPQ(false, true); else VISS_BPQ(false, false);
+    }
     VISS_LAUNCHED(k_backproject_quads);
     if (sky->nrtile > 0) {
       // few tiles (the equator ring of HEALPix): shorter row chunks, so that
@@ -814,28 +829,32 @@ int sky_backproject_ws(viss_sky* sky, const double* uvw, const double* xyza, con
       rpc = std::min(std::max<int64_t>(rpc, 64), kpc);
       if (cdiv(K, rpc) > 65535) rpc = cdiv(cdiv(K, 65535), 64) * 64;
       dim3 rgrid((unsigned)cdiv(sky->nrtile, 4), (unsigned)(chunks ? nchunks : cdiv(K, rpc)));
-      if (mask)
-        k_backproject_pairs<true><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm, vc,
-                                                        K, rpc, herm, x, rc);
-      else
-        k_backproject_pairs<false><<<rgrid, 256, 0, s>>>(sky->rpack, sky->ridx, sky->rcone, sky->nrtile, hot, vm,
-                                                         vc, K, rpc, herm, x, rc);
+      if (mask) {
+        if (chunks) VISS_BPP(rgrid, sky->rpack, sky->ridx, sky->rcone, sky->nrtile, rpc, true, true);
+        else VISS_BPP(rgrid, sky->rpack, sky->ridx, sky->rcone, sky->nrtile, rpc, true, false);
+      } else {
+        if (chunks) VISS_BPP(rgrid, sky->rpack, sky->ridx, sky->rcone, sky->nrtile, rpc, false, true);
+        else VISS_BPP(rgrid, sky->rpack, sky->ridx, sky->rcone, sky->nrtile, rpc, false, false);
+      }
       VISS_LAUNCHED(k_backproject_pairs);
     }
     return VISS_OK;
   }
   if (pairs) {
-    if (mask)
-      k_backproject_pairs<true><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc, K,
-                                                     kpc, herm, x, rc);
-    else
-      k_backproject_pairs<false><<<grid, 256, 0, s>>>(sky->ppack, sky->pidx, sky->pcone, sky->nptile, hot, vm, vc,
-                                                      K, kpc, herm, x, rc);
+    if (mask) {
+      if (chunks) VISS_BPP(grid, sky->ppack, sky->pidx, sky->pcone, sky->nptile, kpc, true, true);
+      else VISS_BPP(grid, sky->ppack, sky->pidx, sky->pcone, sky->nptile, kpc, true, false);
+    } else {
+      if (chunks) VISS_BPP(grid, sky->ppack, sky->pidx, sky->pcone, sky->nptile, kpc, false, true);
+      else VISS_BPP(grid, sky->ppack, sky->pidx, sky->pcone, sky->nptile, kpc, false, false);
+    }
   } else if (mask) {
     if (f64) VISS_BP(true, true); else VISS_BP(true, false);
   } else {
     if (f64) VISS_BP(false, true); else VISS_BP(false, false);
   }
+#undef VISS_BPQ
+#undef VISS_BPP
 #undef VISS_BP
   VISS_LAUNCHED(k_backproject);
   return VISS_OK;
