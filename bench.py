@@ -117,9 +117,10 @@ def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
     ff = kib / max(copies_f)
     out = [f"calibration: {calib_mib} MiB copy reads FETCH_SIZE={max(copies_f):.1f} KiB, WRITE_SIZE={max(copies_w):.1f} KiB",
            f"scale factors: fetch x{ff:.4f}, write x{fw:.4f}"]
-    keys = ("k_forward_quads<true>", "k_backproject_quads<true>", "k_forward_pairs<true>",
-            "k_backproject_pairs<true>", "k_forward<true, false>",
-            "k_backproject<true, false>", "k_forward_snap_pairs<true, 10>", "k_backproject_snap_pairs<true, 10>",
+    # back-projection kernels: <MASK, CHUNKED> (<MASK, F64, CHUNKED> per pixel); both row-chunk variants
+    keys = ("k_forward_quads<true>", "k_backproject_quads<true,", "k_forward_pairs<true>",
+            "k_backproject_pairs<true,", "k_forward<true, false>",
+            "k_backproject<true, false,", "k_forward_snap_pairs<true, 10>", "k_backproject_snap_pairs<true, 10>",
             "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_quads<true>",
             "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
             "k_pix2ang", "k_pixel_dirs",
@@ -585,20 +586,22 @@ def batched(ops, geo, dev):
     st0 = batch.stats()
     side = torch.cuda.Stream()
 
-    def timed(fn, reps=3):
+    def timed(fn, label, reps=3):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             fn()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / reps
+        t = (time.perf_counter() - t0) / reps
+        log(f"[bench] batched: {label} timed ({t * 1e3:.1f} ms)")  # progress (a counter run serialises ~1e4 launches)
+        return t
 
-    t_group = timed(per_group)
+    t_group = timed(per_group, "per-group forward")
     V.zero_()
     per_group()
     V_group = V.clone()
-    t_default = timed(lambda: batch.forward(I, out=V))
+    t_default = timed(lambda: batch.forward(I, out=V), "batch forward")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch.autotune(I, reps=1)
@@ -606,7 +609,7 @@ def batched(ops, geo, dev):
     t_tune = time.perf_counter() - t0
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        t_graph = timed(lambda: batch.forward(I, out=V, use_graph=True))
+        t_graph = timed(lambda: batch.forward(I, out=V, use_graph=True), "tuned batch forward (graph)")
         V.zero_()
         batch.forward(I, out=V, use_graph=True)
     torch.cuda.synchronize()
@@ -635,10 +638,10 @@ def batched(ops, geo, dev):
         x_b.zero_()
         batch.backproject(yv, w, out=x_b, use_graph=True)
 
-    t_img_group = timed(img_group)
+    t_img_group = timed(img_group, "per-group imaging")
     with torch.cuda.stream(side):
         side.wait_stream(torch.cuda.current_stream())
-        t_img_batch = timed(img_batch)
+        t_img_batch = timed(img_batch, "batch imaging")
     torch.cuda.synchronize()
     rel_img = float(torch.linalg.vector_norm(x_b - x_g) / torch.linalg.vector_norm(x_g))
     log(f"[bench] batched imaging: {t_img_batch * 1e3:.1f} ms vs per-group {t_img_group * 1e3:.1f} ms, "

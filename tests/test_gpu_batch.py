@@ -2,12 +2,16 @@
 with uneven compacted tile lists in one launch, eager and hipGraph-replayed,
 against per-group launches and the FP64 oracle.
 
+A HEALPix sky turned off its axes keeps its antipodal pairs but loses the
+equatorial-mirror quads: there the batch runs its pair-tile kernel, checked
+the same way, with the batched back-projection beside it.
+
 Tolerances: batched vs per-group launches within 1e-12 relative L2 for the
 all-FP64 kernels (the same kernel on the same tiles; only the FP64 combine
-order differs) and 1e-6 for mixed precision (per-group mixed launches on a
-paired sky use the antipodal-pair kernel, the batch the per-pixel one: two
-FP32 formulations); eager vs graph-replayed batches within 1e-12; against the
-FP64 oracle within 1e-5 (mixed) / 1e-11 (FP64).
+order differs) and 1e-6 for mixed precision (the same FP32 tile formulations;
+FP32 partial sums combine in a different order); eager vs graph-replayed
+batches within 1e-12; against the FP64 oracle within 1e-5 (mixed) / 1e-11
+(FP64).
 """
 import numpy as np
 import pytest
@@ -86,3 +90,52 @@ def test_batch_graph_replays_and_autotune(setup, tmp_path):
     b2.load_tuning(path)
     V3 = b2.forward(I)
     assert rel_l2(V3.cpu().numpy(), eager.cpu().numpy()) < 1e-12
+
+
+def test_batch_over_pairs_without_quads(setup):
+    """Rotated nside-32 HEALPix sky: pairs, no quads; batch forward (eager and graph) and
+    back-projection against per-group launches, forward rows against the oracle."""
+    ops, _, _, uvw, xa, xb, offs, lmn0, host = setup
+    a, b = 0.3, 0.7
+    rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    lmn_h = (rx @ rz) @ lmn0.cpu().numpy()
+    lmn = torch.from_numpy(np.ascontiguousarray(lmn_h)).cuda()
+    sky = ops.Sky(lmn)
+    lay = sky.pairs()
+    print(f"rotated nside 32: {lay}", flush=True)
+    assert lay["uses_pairs"] and not lay["uses_quads"] and lay["singles"] == 0
+    rng = np.random.default_rng(21)
+    I = torch.from_numpy(rng.lognormal(0.0, 1.0, sky.P)).cuda()
+    batch = ops.Batch(sky, offs, uvw, xa, xb, mask=True, R_moon=R)
+    V = batch.forward(I)
+    G = len(offs) - 1
+    Vg = torch.cat([sky.forward(I, uvw[:, offs[g]:offs[g + 1]], xa[:, offs[g]:offs[g + 1]],
+                                xb[:, offs[g]:offs[g + 1]], mask=True, R_moon=R) for g in range(G)])
+    e = rel_l2(V.cpu().numpy(), Vg.cpu().numpy())
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    out = torch.zeros_like(V)
+    with torch.cuda.stream(side):
+        for rep in range(2):
+            out.zero_()
+            batch.forward(I, out=out, use_graph=True)
+            side.synchronize()
+            assert rel_l2(out.cpu().numpy(), V.cpu().numpy()) < 1e-12, rep
+    torch.cuda.synchronize()
+    rows = np.r_[0:200, offs[-2]:offs[-1]]
+    ref = oracle.forward(*lmn_h, I.cpu().numpy(), host[0][:, rows], host[1][:, rows], host[2][:, rows],
+                         flags=oracle.MASK, R_moon=R)
+    eo = rel_l2(V.cpu().numpy()[rows], ref)
+    y = torch.from_numpy(rng.standard_normal(V.numel()) + 1j * rng.standard_normal(V.numel())).cuda()
+    wt = torch.from_numpy(rng.uniform(0.5, 1.5, V.numel())).cuda()
+    x = batch.backproject(y, wt)
+    xg = torch.zeros(sky.P, dtype=torch.float64, device="cuda")
+    for g in range(G):
+        s = slice(offs[g], offs[g + 1])
+        sky.backproject(uvw[:, s].contiguous(), y[s].contiguous(), wt[s].contiguous(), xa[:, s].contiguous(),
+                        xb[:, s].contiguous(), mask=True, R_moon=R, out=xg)
+    ex = rel_l2(x.cpu().numpy(), xg.cpu().numpy())
+    print(f"pairs batch: forward vs per-group {e:.2e}, vs oracle {eo:.2e}; back-projection vs per-group {ex:.2e}",
+          flush=True)
+    assert e < 1e-6 and eo < 1e-5 and ex < 1e-6

@@ -188,9 +188,11 @@ __device__ __forceinline__ void bp_snapshot(const SatRec* __restrict__ R, const 
 // satellites per snapshot (padding rows carry y = 0), so the pair loops have
 // no per-satellite branches.  Snapshots are classified against the tile 64
 // at a time, one per lane; those every satellite sees whole skip the
-// per-pixel occultation test, those no satellite sees are skipped.
+// per-pixel occultation test, those no satellite sees are skipped.  Held to 6
+// wavefronts per SIMD up to NMAX 12 (the masked kernels otherwise get 81-97
+// VGPRs, 4-5 wavefronts; at NMAX 16 six would spill VGPRs to scratch).
 template <bool MASK, int NMAX>
-__global__ __launch_bounds__(256) void k_backproject_snap(const PixPack* __restrict__ pix,
+__global__ __launch_bounds__(256, NMAX <= 12 ? 6 : 1) void k_backproject_snap(const PixPack* __restrict__ pix,
                                                           const float4* __restrict__ bcone, int64_t nbtile,
                                                           const SatRec* __restrict__ rec,
                                                           const SatCls* __restrict__ cls,
@@ -497,8 +499,11 @@ __device__ __forceinline__ void bp_snapshot_pair(const SatRec* __restrict__ R, c
 // The adjoint over a paired sky: one wavefront per 64-pair tile, one pair per
 // lane; snapshots classified 64 at a time (one per lane) against both sides of
 // the tile and run in the four mode loops.
+// Held to 6 wavefronts per SIMD up to NMAX 12, as k_backproject_snap: at 10
+// satellites, 900,000 rows x 786,432 pixels, 71.8 ms against 75.4 ms with the
+// masked kernel's own 81 VGPRs (5 wavefronts).
 template <bool MASK, int NMAX>
-__global__ __launch_bounds__(256) void k_backproject_snap_pairs(const PairPack* __restrict__ pp,
+__global__ __launch_bounds__(256, NMAX <= 12 ? 6 : 1) void k_backproject_snap_pairs(const PairPack* __restrict__ pp,
                                                                 const int2* __restrict__ pidx,
                                                                 const float4* __restrict__ pcone, int64_t ntile,
                                                                 const SatRec* __restrict__ rec,
