@@ -9,7 +9,9 @@ need the oracle to finish the whole problem:
   * precision, nside 256, 900,000 visibilities (10 satellites x 20,000
     snapshots, 10 MHz, sigma 50 km): the mixed kernels against the all-FP64
     kernels over every row and pixel (7.1e11 pair evaluations each way), the
-    FP64 forward against the oracle on 256 rows, and the adjoint identity
+    FP64 and the mixed forward against the oracle on 2,048 sampled rows, the
+    mixed density-compensated image of those rows against the oracle's (weights
+    over all rows), and the adjoint identity
     <A x, y> = <x, A^H y> of the mixed pair at full size; and the occultation
     decisions of the mixed quad kernels, exactly, through visible-pixel counts
     (zero baselines, unit inputs) against the all-FP64 kernels;
@@ -94,12 +96,23 @@ def test_precision_workload_full_size(ops):
     V64 = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R, fp64=True)
     V32 = sky.forward(I, uvw, xa, xb, mask=True, R_moon=R)
     rv = float(torch.linalg.vector_norm(V32 - V64) / torch.linalg.vector_norm(V64))
-    # the FP64 kernel against the oracle on a row sample
-    sel = np.sort(np.random.default_rng(11).choice(K, 256, replace=False))
+    # the FP64 and the mixed kernels against the oracle on a row sample
+    sel = np.sort(np.random.default_rng(11).choice(K, 2048, replace=False))
+    sel_t = torch.from_numpy(sel).cuda()
     l, m, n = (lmn[i].cpu().numpy() for i in range(3))
-    Vo = oracle.forward(l, m, n, I.cpu().numpy(), inp.uvw[:, sel], inp.xyz_a[:, sel], inp.xyz_b[:, sel],
-                        flags=oracle.MASK, R_moon=R)
-    ro = rel_l2(V64[torch.from_numpy(sel).cuda()].cpu().numpy(), Vo)
+    u, a, b = inp.uvw[:, sel], inp.xyz_a[:, sel], inp.xyz_b[:, sel]
+    Vo = oracle.forward(l, m, n, I.cpu().numpy(), u, a, b, flags=oracle.MASK, R_moon=R)
+    ro = rel_l2(V64[sel_t].cpu().numpy(), Vo)
+    rm = rel_l2(V32[sel_t].cpu().numpy(), Vo)
+    # density-compensated image of the sampled rows (weights over all rows): the mixed V through the
+    # mixed kernels against the oracle's V through the oracle's back-projection
+    bll = inp.bll.astype(np.float32)
+    nr = int(oracle.shell_index(bll).max())
+    wt = ops.weights(torch.from_numpy(bll).cuda(), uvw[2].contiguous(), nr, want_dg=False)["wt"]
+    xs = sky.backproject(uvw[:, sel_t].contiguous(), V32[sel_t].contiguous(), wt[sel_t].contiguous(),
+                         xa[:, sel_t].contiguous(), xb[:, sel_t].contiguous(), mask=True, R_moon=R)
+    xo = oracle.backproject(l, m, n, u, Vo, wt[sel_t].cpu().numpy(), a, b, flags=oracle.MASK, R_moon=R)
+    ri = rel_l2(xs.cpu().numpy(), xo)
     # back-projection of the visibilities, mixed vs FP64
     x64 = sky.backproject(uvw, V64, None, xa, xb, mask=True, R_moon=R, fp64=True)
     x32 = sky.backproject(uvw, V64, None, xa, xb, mask=True, R_moon=R)
@@ -113,10 +126,11 @@ def test_precision_workload_full_size(ops):
     lhs = float(torch.real(torch.sum(Ax * torch.conj(yr))))
     rhs = float(torch.dot(xr, AHy))
     rd = abs(lhs - rhs) / abs(rhs)
-    print(f"  forward mixed vs FP64 {rv:.2e}; FP64 vs oracle {ro:.2e} (256 rows); back-projection mixed vs FP64 "
-          f"{rx:.2e}; adjoint identity {rd:.2e}", flush=True)
+    print(f"  forward mixed vs FP64 {rv:.2e}; against the oracle on 2,048 rows: FP64 {ro:.2e}, mixed {rm:.2e}, "
+          f"mixed density-compensated image {ri:.2e}; back-projection mixed vs FP64 {rx:.2e}; adjoint identity "
+          f"{rd:.2e}", flush=True)
     assert rv <= 1e-5 and rx <= 1e-5
-    assert ro <= 1e-11
+    assert ro <= 1e-11 and rm <= 1e-5 and ri <= 1e-5
     assert rd <= 1e-6
 
 

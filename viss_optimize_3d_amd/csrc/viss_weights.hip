@@ -332,8 +332,11 @@ extern "C" int viss_weights(const float* bll, const double* w, int64_t K, int32_
   const size_t tab = ((size_t)nr + 1) * sizeof(double);
   // the row kernel builds the table in LDS when it fits and the rows are many
   // enough to pay for one table per block; otherwise a launch of its own writes
-  // it (also for K = 0: the table of an empty histogram)
-  const bool lds_tab = K > 0 && tab <= (size_t)lds_max && K >= 16 * ((int64_t)nr + 1);
+  // it (also for K = 0: the table of an empty histogram) and the rows gather it
+  // from L2.  Measured with 1024-thread blocks: 4e7 rows over 16,012 shells,
+  // 171 us from LDS against 330 us from L2; 1.8e6 rows over 11,082 shells (162
+  // rows per shell), 13.5 us from L2 against 15.6 us from LDS.
+  const bool lds_tab = K > 0 && tab <= (size_t)lds_max && K >= 512 * ((int64_t)nr + 1);
   if (!lds_tab) {
     int64_t g = ((int64_t)nr + 1 + 255) / 256;
     if (g > 4096) g = 4096;
