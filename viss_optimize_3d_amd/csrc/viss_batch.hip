@@ -236,11 +236,11 @@ __global__ __launch_bounds__(256) void k_forward_batch_pairs(const PairPack* __r
 // The same work table over the quad layout (mixed precision): list entries
 // below nqtile are 16-quad tiles, the others tiles of the rest layout.  Both
 // tile loops are inlined here, and left to itself the register allocator
-// gives the kernel 103 VGPRs (4 wavefronts per SIMD); held to 8 wavefronts it
-// fits in 56 without spilling and runs 1.18x faster at 4.48e6 rows x 196,608
-// pixels (5.65e12 against 4.79e12 pairs/s).
+// gives the kernel over 100 VGPRs (4 wavefronts per SIMD).  Held to 7
+// wavefronts it fits in 66 without spilling (8 would spill to scratch): at
+// 4.48e6 rows x 196,608 pixels, 102.9 ms per pass against 128 ms at 4.
 template <bool MASK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_forward_batch_quads(const QuadPack* __restrict__ qp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_forward_batch_quads(const QuadPack* __restrict__ qp,
                                                              const double* __restrict__ qz,
                                                              const float4* __restrict__ qcone, int64_t nqtile,
                                                              const PairPack* __restrict__ rp,
@@ -259,14 +259,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   const int64_t k = w.row0 + lane;
   const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, Ktot, k, lane < w.nrows, R_moon, orbit_r, track);
   const int32_t* __restrict__ list = tiles + (int64_t)w.g * ntile;
+  const uint64_t actm = wave_ballot(r.act);
+  if (MASK && actm == 0) return;  // wave-uniform: no row in this item
+  WaveCls wb{};
+  if (MASK) wb = wave_cls(r.cl, r.act);
   double Sr = 0.0, Si = 0.0;
-  for (int64_t j = w.j0; j < w.j1; ++j) {
-    const int64_t t = list[j];  // wave-uniform
-    if (t < nqtile)
-      forward_quad_tiles<MASK>(qp, qz, qcone, t, t + 1, r, Sr, Si);
-    else
-      forward_pair_tiles<MASK>(rp, rcone, nullptr, t - nqtile, t - nqtile + 1, r, Sr, Si);
+  QuadSums qs;
+  // list entries 64 at a time: each lane classifies one quad tile against the
+  // item's bounds (as forward_quad_tiles), then the wavefront walks them.
+  // Each tile's FP32 sum joins the FP64 sums at once, so results differ
+  // between work tables (autotuned or not) by FP64 rounding only.
+  for (int64_t jb = w.j0; jb < w.j1; jb += 64) {
+    const int64_t je = min(w.j1, jb + 64);
+    int code = kTileExact;
+    if (MASK && jb + lane < je) {
+      const int64_t e = list[jb + lane];
+      if (e < nqtile) code = quad_tile_code(qcone[2 * e], qcone[2 * e + 1], wb);
+    }
+    for (int64_t j = jb; j < je; ++j) {
+      const int64_t t = list[j];  // wave-uniform
+      if (t < nqtile) {
+        int mz = kRingFF, mm = kRingFF;
+        if (MASK) {
+          quad_tile_modes(qcone, t, __builtin_amdgcn_readlane(code, (int)(j - jb)), r, actm, mz, mm);
+          if (mz == kRingNone && mm == kRingNone) continue;
+        }
+        quad_tile<MASK>(qp, qz, t, mz, mm, r, qs, Sr, Si);
+        qs.flush(Sr, Si);  // per tile: the sums do not depend on how the table cuts the list
+      } else {
+        forward_pair_tiles<MASK>(rp, rcone, nullptr, t - nqtile, t - nqtile + 1, r, Sr, Si);
+      }
+    }
   }
+  qs.flush(Sr, Si);
   if (r.act) forward_pair_store(V, k, Ktot, 0, r.c0, Sr, Si);
 }
 

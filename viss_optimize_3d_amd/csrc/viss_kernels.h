@@ -419,48 +419,97 @@ __device__ __forceinline__ void ring_mode_masks(uint64_t any1, uint64_t all1, ui
   m[kRingNF] = ~any1 & any2 & untested;
 }
 
-// Accumulates the quad tiles [t0, t1) into (Sr, Si) (the convention of
-// forward_store, without the phase-tracking term).  The ring-factor products
-// of up to 8 tiles are summed in FP32 before they join the FP64 sums.
-template <bool MASK>
-__device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
-                                                   const float4* __restrict__ qcone, int64_t t0, int64_t t1,
-                                                   const FwdRow& r, double& Sr, double& Si) {
+// Bounds of a wavefront's rows for classifying tiles once for all of them:
+// the mean unit direction of each satellite over the active lanes, the
+// largest distance of a lane's direction from it, and the range of the unit
+// cuts.  For a unit pixel s, |s.a_k - s.a| <= |a_k - a| <= da, so a tile side
+// the bounds show unseen (whole) is unseen (whole) for every lane as well.
+// The margin on da covers the FP32 rounding of the bounds against that of the
+// per-lane tests of classify_pair_masks.
+struct WaveCls {
+  float ax, ay, az, da, camin, camax;
+  float bx, by, bz, db, cbmin, cbmax;
+};
+
+__device__ __forceinline__ WaveCls wave_cls(const VisCls& c, bool act) {
+  float v[6] = {act ? c.ax : 0.f, act ? c.ay : 0.f, act ? c.az : 0.f,
+                act ? c.bx : 0.f, act ? c.by : 0.f, act ? c.bz : 0.f};
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i] += __shfl_xor(v[i], o, 64);
+  const float na = rsqrtf(fmaxf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2], 1e-30f));
+  const float nb = rsqrtf(fmaxf(v[3] * v[3] + v[4] * v[4] + v[5] * v[5], 1e-30f));
+  WaveCls w;
+  w.ax = v[0] * na; w.ay = v[1] * na; w.az = v[2] * na;
+  w.bx = v[3] * nb; w.by = v[4] * nb; w.bz = v[5] * nb;
+  const float ex = c.ax - w.ax, ey = c.ay - w.ay, ez = c.az - w.az;
+  const float fx = c.bx - w.bx, fy = c.by - w.by, fz = c.bz - w.bz;
+  float m[6] = {act ? sqrtf(ex * ex + ey * ey + ez * ez) : 0.f, act ? sqrtf(fx * fx + fy * fy + fz * fz) : 0.f,
+                act ? c.ca : 3.f, act ? -c.ca : -3.f, act ? c.cb : 3.f, act ? -c.cb : -3.f};  // max of each
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[i] = fmaxf(m[i], __shfl_xor(m[i], o, 64));
+  w.da = m[0] + 4e-6f;
+  w.db = m[1] + 4e-6f;
+  w.camax = m[2]; w.camin = -m[3];
+  w.cbmax = m[4]; w.cbmin = -m[5];
+  // every lane holds the same values after the butterflies: keep them in scalar registers
+  const auto u = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+  return WaveCls{u(w.ax), u(w.ay), u(w.az), u(w.da), u(w.camin), u(w.camax),
+                 u(w.bx), u(w.by), u(w.bz), u(w.db), u(w.cbmin), u(w.cbmax)};
+}
+
+// The ring modes forward_quad_tiles takes for a quad tile (cones c0: a and d
+// pixels, c1: b and c), from the wavefront bounds, as (mz << 3) | mm; or
+// kTileExact when the bounds leave a side between unseen and whole.  Where
+// every side is unseen or whole by the bounds, it is so for every lane, and
+// the per-lane masks give the same modes.
+constexpr int kTileExact = 64;
+
+__device__ __forceinline__ int quad_tile_code(const float4 c0, const float4 c1, const WaveCls& b) {
+  bool see[2][2], whole[2][2];  // [cone][side A / side B]
+  const float4 cs[2] = {c0, c1};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const float4 cn = cs[h];
+    const float W = cn.w + kClsEps;
+    const float pa = fmaf(cn.x, b.ax, fmaf(cn.y, b.ay, cn.z * b.az));
+    const float pb = fmaf(cn.x, b.bx, fmaf(cn.y, b.by, cn.z * b.bz));
+    see[h][0] = (pa + b.da - b.camin >= -W) & (pb + b.db - b.cbmin >= -W);
+    whole[h][0] = (pa - b.da - b.camax >= W) & (pb - b.db - b.cbmax >= W);
+    see[h][1] = (pa - b.da + b.camin <= W) & (pb - b.db + b.cbmin <= W);
+    whole[h][1] = (pa + b.da + b.camax <= -W) & (pb + b.db + b.cbmax <= -W);
+  }
+  const int mz = ring_mode(see[0][0], whole[0][0], see[1][0], whole[1][0]);
+  const int mm = ring_mode(see[1][1], whole[1][1], see[0][1], whole[0][1]);
+  return (mz >= kRingTF || mm >= kRingTF) ? kTileExact : (mz << 3) | mm;
+}
+
+// Ring modes of quad tile t from the per-lane masks (ring z: a, b; ring -z:
+// c, d), or those a code of quad_tile_code carries.
+__device__ __forceinline__ void quad_tile_modes(const float4* __restrict__ qcone, int64_t t, int code, const FwdRow& r,
+                                                uint64_t actm, int& mz, int& mm) {
+  if (code == kTileExact) {
+    const PairSides ad = classify_pair_wave(qcone[2 * t], r.cl, actm);      // a and its antipode d
+    const PairSides bc = classify_pair_wave(qcone[2 * t + 1], r.cl, actm);  // b and its antipode c
+    mz = ring_mode(ad.seeA, ad.wholeA, bc.seeA, bc.wholeA);
+    mm = ring_mode(bc.seeB, bc.wholeB, ad.seeB, ad.wholeB);
+  } else {
+    mz = code >> 3;
+    mm = code & 7;
+  }
+}
+
+// Running sums of the quad tiles of one row: the ring-factor products of up
+// to 8 tiles are summed in FP32 before they join the FP64 sums.
+struct QuadSums {
   float fr = 0.f, fi = 0.f;
   int nacc = 0;
-  const uint64_t actm = wave_ballot(r.act);
-  for (int64_t t = t0; t < t1; ++t) {
-    const double z = qz[t];
-    int mz = kRingFF, mm = kRingFF;
-    QuadOcc zz{0.0, 0.0};
-    if (MASK) {
-      const PairSides ad = classify_pair_wave(qcone[2 * t], r.cl, actm);      // a and its antipode d
-      const PairSides bc = classify_pair_wave(qcone[2 * t + 1], r.cl, actm);  // b and its antipode c
-      mz = ring_mode(ad.seeA, ad.wholeA, bc.seeA, bc.wholeA);
-      mm = ring_mode(bc.seeB, bc.wholeB, ad.seeB, ad.wholeB);
-      if (mz == kRingNone && mm == kRingNone) continue;
-      zz.zza = z * r.mk.za;
-      zz.zzb = z * r.mk.zb;
-    }
-    const QuadPack* __restrict__ tq = qp + t * kQuadTile;
-    float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
-#define VISS_QT(A, B)                                                                      \
-  case A * kRingModes + B:                                                                 \
-    quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, zz, Rz, Jz, Rm, Jm);                      \
-    break;
-#define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
-                       VISS_QT(A, kRingTF) VISS_QT(A, kRingFT) VISS_QT(A, kRingTest)
-    switch (mz * kRingModes + mm) {  // wave-uniform
-      VISS_QT(kRingNone, kRingFF) VISS_QT(kRingNone, kRingFN) VISS_QT(kRingNone, kRingNF)
-      VISS_QT(kRingNone, kRingTF) VISS_QT(kRingNone, kRingFT) VISS_QT(kRingNone, kRingTest)
-      VISS_QT_ROW(kRingFF) VISS_QT_ROW(kRingFN) VISS_QT_ROW(kRingNF) VISS_QT_ROW(kRingTF) VISS_QT_ROW(kRingFT)
-      VISS_QT_ROW(kRingTest)
-      default: break;
-    }
-#undef VISS_QT_ROW
-#undef VISS_QT
-    float sc, cc;
-    sincos_rev(rev_fp32(r.w * z), sc, cc);
+  // adds the tile's e^{-i C}(Rz - i Jz) + e^{+i C}(Rm - i Jm), C's sin / cos = (sc, cc)
+  __device__ __forceinline__ void add(float sc, float cc, float Rz, float Jz, float Rm, float Jm, double& Sr,
+                                      double& Si) {
     fr = fmaf(cc, Rz + Rm, fmaf(sc, Jm - Jz, fr));
     fi = fmaf(sc, Rz - Rm, fmaf(cc, Jz + Jm, fi));
     if (++nacc == 8) {
@@ -470,8 +519,77 @@ __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ 
       nacc = 0;
     }
   }
-  Sr += (double)fr;
-  Si += (double)fi;
+  __device__ __forceinline__ void flush(double& Sr, double& Si) {
+    Sr += (double)fr;
+    Si += (double)fi;
+    fr = fi = 0.f;
+    nacc = 0;
+  }
+};
+
+// Quad tile t in ring modes (mz, mm) (not both kRingNone), into qs.
+template <bool MASK>
+__device__ __forceinline__ void quad_tile(const QuadPack* __restrict__ qp, const double* __restrict__ qz, int64_t t,
+                                          int mz, int mm, const FwdRow& r, QuadSums& qs, double& Sr, double& Si) {
+  const double z = qz[t];
+  QuadOcc zz{0.0, 0.0};
+  if (MASK) {
+    zz.zza = z * r.mk.za;
+    zz.zzb = z * r.mk.zb;
+  }
+  const QuadPack* __restrict__ tq = qp + t * kQuadTile;
+  float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
+#define VISS_QT(A, B)                                                                      \
+  case A * kRingModes + B:                                                                 \
+    quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, zz, Rz, Jz, Rm, Jm);                      \
+    break;
+#define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
+                       VISS_QT(A, kRingTF) VISS_QT(A, kRingFT) VISS_QT(A, kRingTest)
+  switch (mz * kRingModes + mm) {  // wave-uniform
+    VISS_QT(kRingNone, kRingFF) VISS_QT(kRingNone, kRingFN) VISS_QT(kRingNone, kRingNF)
+    VISS_QT(kRingNone, kRingTF) VISS_QT(kRingNone, kRingFT) VISS_QT(kRingNone, kRingTest)
+    VISS_QT_ROW(kRingFF) VISS_QT_ROW(kRingFN) VISS_QT_ROW(kRingNF) VISS_QT_ROW(kRingTF) VISS_QT_ROW(kRingFT)
+    VISS_QT_ROW(kRingTest)
+    default: break;
+  }
+#undef VISS_QT_ROW
+#undef VISS_QT
+  float sc, cc;
+  sincos_rev(rev_fp32(r.w * z), sc, cc);
+  qs.add(sc, cc, Rz, Jz, Rm, Jm, Sr, Si);
+}
+
+// Accumulates the quad tiles [t0, t1) into (Sr, Si) (the convention of
+// forward_store, without the phase-tracking term).  With the mask the lanes
+// first classify 64 tiles at a time, one each, against the wavefront's
+// bounds (quad_tile_code); only the tiles the bounds leave open take the
+// per-lane classification (about 34 vector instructions a tile, the bounds
+// under one).  The modes, and so the results, are those of the per-lane
+// classification alone.
+template <bool MASK>
+__device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
+                                                   const float4* __restrict__ qcone, int64_t t0, int64_t t1,
+                                                   const FwdRow& r, double& Sr, double& Si) {
+  const uint64_t actm = wave_ballot(r.act);
+  if (MASK && actm == 0) return;  // no row: every tile would be unseen
+  WaveCls wb{};
+  if (MASK) wb = wave_cls(r.cl, r.act);
+  const int lane = threadIdx.x & 63;
+  QuadSums qs;
+  for (int64_t tb = t0; tb < t1; tb += 64) {
+    const int64_t te = min(t1, tb + 64);
+    int code = kTileExact;
+    if (MASK && tb + lane < te) code = quad_tile_code(qcone[2 * (tb + lane)], qcone[2 * (tb + lane) + 1], wb);
+    for (int64_t t = tb; t < te; ++t) {
+      int mz = kRingFF, mm = kRingFF;
+      if (MASK) {
+        quad_tile_modes(qcone, t, __builtin_amdgcn_readlane(code, (int)(t - tb)), r, actm, mz, mm);
+        if (mz == kRingNone && mm == kRingNone) continue;
+      }
+      quad_tile<MASK>(qp, qz, t, mz, mm, r, qs, Sr, Si);
+    }
+  }
+  qs.flush(Sr, Si);
 }
 
 // Applies the phase-tracking term, V e^{-2 pi i c0} = (Sr - i Si)(cos - i sin),
