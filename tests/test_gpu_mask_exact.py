@@ -13,7 +13,10 @@ baselines exercise the same decisions as real ones.
 Satellite geometries: a tight cluster (10 satellites, sigma 50 km, as the
 precision workload: the wavefront's rows share their cap edges, so tiles take
 the one-side-tested ring modes) and a scattered one (sigma 500 km: many edges
-per wavefront, tiles tested on both sides).  Skies: HEALPix RING and NEST
+per wavefront, tiles tested on both sides).  The quad forward first settles
+tiles against bounds of the wavefront's 64 rows; with the rows in random order
+each wavefront holds unrelated snapshots, the bounds settle few tiles, and the
+per-row classification takes the rest.  Skies: HEALPix RING and NEST
 (quad layout), rings of random azimuths (quads), the same with 5% of the
 pixels removed (quads, pairs and singles), random directions (per-pixel
 kernels) and an exact +-s set (pairs only).
@@ -152,3 +155,27 @@ def test_mask_counts_snapshot_and_batched_kernels(ops, kind, geom):
     bad = np.flatnonzero(Vb.real != rows)
     assert bad.size == 0, f"batched: {bad.size} rows differ"
     print(f"{kind} {geom}: snapshot and batched counts exact over K={K} P={P}", flush=True)
+
+
+@pytest.mark.parametrize("kind", ["healpix32", "rings-minus"])
+def test_mask_counts_rows_shuffled(ops, kind):
+    """Rows in random order through the per-row and batched forward: exact counts."""
+    rng = np.random.default_rng(zlib.crc32(f"{kind}/shuffled".encode()))
+    lmn = sky(kind, rng)
+    pos = positions("cluster", 5)
+    _, xa, xb, _ = geo.baseline_rows(pos, 1.0)
+    perm = rng.permutation(xa.shape[1])
+    xa, xb = xa[:, perm], xb[:, perm]
+    K, P = xa.shape[1], lmn.shape[1]
+    rows, _ = oracle_counts(lmn, xa, xb)
+    s = ops.Sky(torch.from_numpy(lmn).cuda())
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    zero, ta, tb = d(np.zeros((3, K))), d(xa), d(xb)
+    ones_p = torch.ones(P, dtype=torch.float64, device="cuda")
+    V = s.forward(ones_p, zero, ta, tb, mask=True, R_moon=R).cpu().numpy()
+    bad = np.flatnonzero(V.real != rows)
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}: {V.real[bad[:5]]} vs {rows[bad[:5]]}"
+    b = ops.Batch(s, [0, K // 2, K], zero, ta, tb, mask=True, R_moon=R)
+    Vb = b.forward(ones_p).cpu().numpy()
+    assert np.array_equal(Vb.real, rows), "batched"
+    print(f"{kind}: shuffled rows, counts exact over K={K} P={P} (layout {s.pairs()})", flush=True)
