@@ -445,8 +445,9 @@ __device__ __forceinline__ WaveCls wave_cls(const VisCls& c, bool act) {
   w.bx = v[3] * nb; w.by = v[4] * nb; w.bz = v[5] * nb;
   const float ex = c.ax - w.ax, ey = c.ay - w.ay, ez = c.az - w.az;
   const float fx = c.bx - w.bx, fy = c.by - w.by, fz = c.bz - w.bz;
+  // maxima over the active lanes (cuts lie in [-2, 1]: -3 is below any of them and of their negations)
   float m[6] = {act ? sqrtf(ex * ex + ey * ey + ez * ez) : 0.f, act ? sqrtf(fx * fx + fy * fy + fz * fz) : 0.f,
-                act ? c.ca : 3.f, act ? -c.ca : -3.f, act ? c.cb : 3.f, act ? -c.cb : -3.f};  // max of each
+                act ? c.ca : -3.f, act ? -c.ca : -3.f, act ? c.cb : -3.f, act ? -c.cb : -3.f};
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
