@@ -2,9 +2,10 @@
 frequencies x 64 groups x 500 snapshots x 28 baselines = 4,480,000 rows, nside
 128), every group's weights and images from one batch:
 
-  * viss_batch_weights against viss_weights on each group's rows alone: gs, mb,
-    dcf, dg and wt bit-exact (the same expressions; only the launch layout
-    differs), and the first and last group against the oracle (check_weights);
+  * viss_batch_weights against viss_weights on each group's rows alone, with
+    and without the blockage indexing of dcf: gs, mb, dcf, dg and wt bit-exact
+    (the same expressions; only the launch layout differs), and the first and
+    last group against the oracle (check_weights);
   * viss_batch_backproject against viss_sky_backproject of each group's rows
     with those weights, summed over groups and per group: within 1e-6 relative
     L2 in mixed precision (the same tile kernels; row chunks and the order of
@@ -56,26 +57,27 @@ def _sample_rows(offs, gf):
     return rows, groups
 
 
-def test_batch_weights_match_per_group(wl):
+@pytest.mark.parametrize("shift", [False, True])
+def test_batch_weights_match_per_group(wl, shift):
     ops, sky, lmn, U, A, Bx, bll, offs, gf, nr, y, host = wl
     batch = ops.Batch(sky, offs, U, A, Bx, mask=True, R_moon=R)
-    out = batch.weights(bll, nr)
+    out = batch.weights(bll, nr, blockage_shift=shift)
     G = len(offs) - 1
     for g in range(G):
         s = slice(offs[g], offs[g + 1])
-        ref = ops.weights(bll[s].contiguous(), U[2, s].contiguous(), nr)
+        ref = ops.weights(bll[s].contiguous(), U[2, s].contiguous(), nr, blockage_shift=shift)
         for key in ("gs", "wt", "dg"):
             assert torch.equal(out[key][s], ref[key]), (g, key)
         assert torch.equal(out["mb"][g], ref["mb"]), g
         assert torch.equal(out["dcf"][g], ref["dcf"]), g
     assert int(out["n_out_of_range"].item()) == 0
-    print(f"batch weights of {G} groups bit-exact with per-group viss_weights", flush=True)
+    print(f"batch weights of {G} groups bit-exact with per-group viss_weights (blockage shift {shift})", flush=True)
     uvw, xa, xb, bllh = host
     for g in (0, G - 1):
         s = slice(offs[g], offs[g + 1])
         sub = {k: (out[k][s] if k in ("gs", "wt", "dg") else out[k][g]) for k in ("gs", "wt", "dg", "mb", "dcf")}
         sub["n_out_of_range"] = torch.zeros(1, dtype=torch.int64)
-        check_weights(sub, bllh[s], uvw[2, s], nr)
+        check_weights(sub, bllh[s], uvw[2, s], nr, shift=shift)
 
 
 @pytest.mark.parametrize("fp64", [False, True])

@@ -244,7 +244,8 @@ class Sky:
 
 
 def weights(bll: torch.Tensor, w: torch.Tensor, nr: int, blockage_shift: bool = False, want_dg: bool = True):
-    """Dict with gs (int32), mb (uint32 as int64 view... stored int32), dcf, dg, wt (float64), n_out_of_range."""
+    """Dict with gs (int32 [K]), mb (the uint32 shell counts in an int32 tensor [nr]), dcf (float64 [nr + 1]),
+    dg and wt (float64 [K]; dg None without want_dg) and n_out_of_range (int64 [1])."""
     L = _lib.load()
     bll = _dev(bll, torch.float32)
     w = _dev(w, torch.float64)
