@@ -179,3 +179,35 @@ def test_mask_counts_rows_shuffled(ops, kind):
     Vb = b.forward(ones_p).cpu().numpy()
     assert np.array_equal(Vb.real, rows), "batched"
     print(f"{kind}: shuffled rows, counts exact over K={K} P={P} (layout {s.pairs()})", flush=True)
+
+
+def test_mask_counts_with_nan_rows(ops):
+    """Rows with a NaN satellite coordinate see no pixel (every comparison with NaN is false,
+    as in the oracle); the other rows of their wavefronts keep their exact counts, through the
+    per-row and the batched forward."""
+    rng = np.random.default_rng(77)
+    lmn = sky("healpix32", rng)
+    pos = positions("cluster", 6)
+    _, xa, xb, _ = geo.baseline_rows(pos, 1.0)
+    K, P = xa.shape[1], lmn.shape[1]
+    bad = np.array([5, 70, 71, 200, K - 1])
+    xa, xb = xa.copy(), xb.copy()
+    xa[0, bad[:2]] = np.nan
+    xb[2, bad[2]] = np.nan
+    xa[1, bad[3]] = np.nan
+    xb[:, bad[4]] = np.nan
+    rows, _ = oracle_counts(lmn, xa, xb)
+    assert np.all(rows[bad] == 0) and np.all(rows[np.setdiff1d(np.arange(K), bad)] > 0)
+    s = ops.Sky(torch.from_numpy(lmn).cuda())
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    zero, ta, tb = d(np.zeros((3, K))), d(xa), d(xb)
+    ones_p = torch.ones(P, dtype=torch.float64, device="cuda")
+    for fp64 in (False, True):
+        V = s.forward(ones_p, zero, ta, tb, mask=True, R_moon=R, fp64=fp64).cpu().numpy()
+        diff = np.flatnonzero(V.real != rows)
+        assert diff.size == 0, f"fp64={fp64}: {diff.size} rows differ, first {diff[:5]}: {V.real[diff[:5]]} vs {rows[diff[:5]]}"
+    b = ops.Batch(s, [0, K // 2, K], zero, ta, tb, mask=True, R_moon=R)
+    Vb = b.forward(ones_p).cpu().numpy()
+    diff = np.flatnonzero(Vb.real != rows)
+    assert diff.size == 0, f"batched: {diff.size} rows differ, first {diff[:5]}"
+    print(f"NaN rows {bad.tolist()}: no pixel; the other {K - bad.size} rows exact", flush=True)

@@ -432,6 +432,15 @@ struct WaveCls {
 };
 
 __device__ __forceinline__ WaveCls wave_cls(const VisCls& c, bool act) {
+  // a row with a non-finite direction or cut (a NaN or infinite satellite
+  // position) fails the per-lane tests; bounds built with it would be NaN and
+  // settle every tile as unseen for all rows, so such a wavefront gets bounds
+  // that settle nothing (every tile takes the per-lane classification)
+  const bool fin = __builtin_isfinite(c.ax) & __builtin_isfinite(c.ay) & __builtin_isfinite(c.az) &
+                   __builtin_isfinite(c.ca) & __builtin_isfinite(c.bx) & __builtin_isfinite(c.by) &
+                   __builtin_isfinite(c.bz) & __builtin_isfinite(c.cb);
+  if (wave_ballot(act & !fin) != 0)  // wave-uniform
+    return WaveCls{0.f, 0.f, 0.f, __builtin_inff(), 0.f, 0.f, 0.f, 0.f, 0.f, __builtin_inff(), 0.f, 0.f};
   float v[6] = {act ? c.ax : 0.f, act ? c.ay : 0.f, act ? c.az : 0.f,
                 act ? c.bx : 0.f, act ? c.by : 0.f, act ? c.bz : 0.f};
 #pragma unroll
