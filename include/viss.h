@@ -230,7 +230,10 @@ int viss_sky_count_work(viss_sky* sky, const double* xyz_a, const double* xyz_b,
  * group gets a compacted list of the sky tiles its satellites can see;
  * the lists are cut into work items (64 rows x a tile range sized per
  * bucket of list length and group rows) and one wavefront runs each item.  flags: VISS_MASK,
- * VISS_PHASE_TRACK, VISS_PREC_FP64.  The batch keeps a pointer to `sky`. */
+ * VISS_PHASE_TRACK, VISS_PREC_FP64.  The batch keeps a pointer to `sky`.
+ * Groups may be empty; with no row at all (K_total = 0) the row arrays, and
+ * the row-sized arguments of the calls below (V, y, wt, bll, gs, dg, V_scratch),
+ * may be NULL, and the calls add nothing. */
 typedef struct viss_batch viss_batch;
 
 int viss_batch_create(viss_sky* sky, int64_t n_groups, const int64_t* row_offsets, const double* uvw,

@@ -139,3 +139,70 @@ def test_batch_over_pairs_without_quads(setup):
     print(f"pairs batch: forward vs per-group {e:.2e}, vs oracle {eo:.2e}; back-projection vs per-group {ex:.2e}",
           flush=True)
     assert e < 1e-6 and eo < 1e-5 and ex < 1e-6
+
+
+def test_batch_with_empty_groups(setup):
+    """Groups without rows (first, middle, last) in a batch: forward (eager and graph), weights
+    and per-group images against per-group calls; an empty group's weights are those of
+    viss_weights on no rows and its image stays 0."""
+    ops, sky, I, uvw, xa, xb, offs, lmn, host = setup
+    K = 350
+    offs2 = [0, 0, 100, 100, 350, 350]
+    U, A, Bx = (t[:, :K].contiguous() for t in (uvw, xa, xb))
+    batch = ops.Batch(sky, offs2, U, A, Bx, mask=True, R_moon=R)
+    V = batch.forward(I)
+    groups = [(offs2[g], offs2[g + 1]) for g in range(len(offs2) - 1)]
+    Vg = torch.zeros_like(V)
+    for k0, k1 in groups:
+        if k1 > k0:
+            sky.forward(I, U[:, k0:k1].contiguous(), A[:, k0:k1].contiguous(), Bx[:, k0:k1].contiguous(), mask=True,
+                        R_moon=R, out=Vg[k0:k1])
+    assert rel_l2(V.cpu().numpy(), Vg.cpu().numpy()) < 1e-6
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    out = torch.zeros_like(V)
+    with torch.cuda.stream(side):
+        batch.forward(I, out=out, use_graph=True)
+        side.synchronize()
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), V.cpu().numpy()) < 1e-12
+    bll = torch.sqrt((U * U).sum(0)).float()
+    nr = int(torch.ceil((bll.max() - 0.25) / 0.5).item()) + 1
+    w = batch.weights(bll, nr)
+    for g, (k0, k1) in enumerate(groups):
+        ref = ops.weights(bll[k0:k1].contiguous(), U[2, k0:k1].contiguous(), nr)
+        assert torch.equal(w["mb"][g], ref["mb"]), g
+        assert torch.equal(w["dcf"][g], ref["dcf"]), g
+        for key in ("gs", "wt", "dg"):
+            assert torch.equal(w[key][k0:k1], ref[key]), (g, key)
+    rng = np.random.default_rng(9)
+    y = torch.from_numpy(rng.standard_normal(K) + 1j * rng.standard_normal(K)).cuda()
+    x = batch.backproject(y, w["wt"], per_group=True)
+    for g, (k0, k1) in enumerate(groups):
+        if k1 == k0:
+            assert float(x[g].abs().max()) == 0.0, g
+        else:
+            xr = sky.backproject(U[:, k0:k1].contiguous(), y[k0:k1].contiguous(), w["wt"][k0:k1].contiguous(),
+                                 A[:, k0:k1].contiguous(), Bx[:, k0:k1].contiguous(), mask=True, R_moon=R)
+            assert rel_l2(x[g].cpu().numpy(), xr.cpu().numpy()) < 1e-6, g
+    print(f"batch with empty groups {[g for g, (a, b) in enumerate(groups) if a == b]}: forward, weights and images "
+          f"as per-group calls", flush=True)
+
+
+def test_batch_without_rows(setup):
+    """A batch whose groups are all empty: the forward returns nothing, the weights are those of
+    viss_weights on no rows, the images stay 0."""
+    ops, sky, I, uvw, xa, xb, offs, lmn, host = setup
+    e = torch.zeros(3, 0, dtype=torch.float64, device="cuda")
+    batch = ops.Batch(sky, [0, 0, 0], e, e, e, mask=True, R_moon=R)
+    V = batch.forward(I)
+    assert V.numel() == 0
+    V = batch.forward(I, out=V, use_graph=False)
+    bll = torch.zeros(0, dtype=torch.float32, device="cuda")
+    w = batch.weights(bll, 8)
+    ref = ops.weights(bll, torch.zeros(0, dtype=torch.float64, device="cuda"), 8)
+    for g in range(2):
+        assert torch.equal(w["mb"][g], ref["mb"]) and torch.equal(w["dcf"][g], ref["dcf"]), g
+    assert int(w["n_out_of_range"].item()) == 0
+    x = batch.backproject(torch.zeros(0, dtype=torch.complex128, device="cuda"), w["wt"], per_group=True)
+    assert tuple(x.shape) == (2, sky.P) and float(x.abs().max()) == 0.0

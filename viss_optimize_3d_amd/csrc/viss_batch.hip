@@ -562,7 +562,7 @@ int viss_batch_create(viss_sky* sky, int64_t n_groups, const int64_t* row_offset
                       viss_batch** out, void* stream) {
   VISS_REQUIRE(out != nullptr, "viss_batch_create: out is NULL");
   *out = nullptr;
-  VISS_REQUIRE(sky != nullptr && row_offsets != nullptr && uvw != nullptr, "viss_batch_create: NULL argument");
+  VISS_REQUIRE(sky != nullptr && row_offsets != nullptr, "viss_batch_create: NULL argument");
   VISS_REQUIRE(n_groups >= 1 && n_groups <= (1 << 20), "viss_batch_create: n_groups %lld out of range",
                (long long)n_groups);
   VISS_REQUIRE((flags & ~(VISS_MASK | VISS_PHASE_TRACK | VISS_PREC_FP64)) == 0,
@@ -574,8 +574,10 @@ int viss_batch_create(viss_sky* sky, int64_t n_groups, const int64_t* row_offset
     VISS_REQUIRE(row_offsets[g + 1] >= row_offsets[g], "viss_batch_create: row_offsets must be non-decreasing");
   const int64_t Ktot = row_offsets[n_groups];
   VISS_REQUIRE(Ktot <= ((int64_t)1 << 40), "viss_batch_create: too many rows");
+  VISS_REQUIRE(Ktot == 0 || uvw != nullptr, "viss_batch_create: uvw is NULL");
   if (flags & VISS_MASK) {
-    VISS_REQUIRE(geom && xyz_a && xyz_b, "viss_batch_create: VISS_MASK needs geometry and satellite positions");
+    VISS_REQUIRE(geom && (Ktot == 0 || (xyz_a && xyz_b)),
+                 "viss_batch_create: VISS_MASK needs geometry and satellite positions");
     VISS_REQUIRE(geom->R_moon >= 0.0 && (geom->orbit_radius <= 0.0 || geom->orbit_radius > geom->R_moon),
                  "viss_batch_create: bad geometry");
   }
@@ -613,9 +615,11 @@ int viss_batch_destroy(viss_batch* b) {
 }
 
 int viss_batch_forward(viss_batch* b, const double* I, double* V, int use_graph, void* stream) {
-  VISS_REQUIRE(b != nullptr && I != nullptr && V != nullptr, "viss_batch_forward: NULL argument");
+  VISS_REQUIRE(b != nullptr && I != nullptr, "viss_batch_forward: NULL argument");
   int rc = check_device(b->sky);
   if (rc != VISS_OK) return rc;
+  if (b->Ktot == 0) return VISS_OK;  // no row: nothing to add (V may be NULL)
+  VISS_REQUIRE(V != nullptr, "viss_batch_forward: V is NULL");
   hipStream_t s = as_stream(stream);
   VISS_REQUIRE(!use_graph || s != nullptr,
                "viss_batch_forward: graph mode needs a non-default stream (capture is not allowed on stream 0)");
@@ -650,10 +654,12 @@ int viss_batch_forward(viss_batch* b, const double* I, double* V, int use_graph,
 }
 
 int viss_batch_autotune(viss_batch* b, const double* I, double* V_scratch, int reps, void* stream) {
-  VISS_REQUIRE(b != nullptr && I != nullptr && V_scratch != nullptr, "viss_batch_autotune: NULL argument");
+  VISS_REQUIRE(b != nullptr && I != nullptr, "viss_batch_autotune: NULL argument");
   VISS_REQUIRE(reps >= 1 && reps <= 100, "viss_batch_autotune: reps %d outside [1, 100]", reps);
   int rc = check_device(b->sky);
   if (rc != VISS_OK) return rc;
+  if (b->Ktot == 0) return VISS_OK;  // no row, no bucket to tune
+  VISS_REQUIRE(V_scratch != nullptr, "viss_batch_autotune: V_scratch is NULL");
   hipStream_t s = as_stream(stream);
   rc = sky_set_brightness(b->sky, I, (b->flags & VISS_PREC_FP64) != 0, s);
   if (rc != VISS_OK) return rc;
@@ -768,12 +774,13 @@ static int launch_batch_bp(viss_batch* b, const double* y, const double* wt, int
 
 int viss_batch_backproject(viss_batch* b, const double* y, const double* wt, int flags, double* x, int per_group,
                            int use_graph, void* stream) {
-  VISS_REQUIRE(b != nullptr && y != nullptr && x != nullptr, "viss_batch_backproject: NULL argument");
+  VISS_REQUIRE(b != nullptr && x != nullptr, "viss_batch_backproject: NULL argument");
   int rc = check_device(b->sky);
   if (rc != VISS_OK) return rc;
   VISS_REQUIRE((flags & ~(VISS_PHASE_TRACK | VISS_HERMITIAN)) == 0, "viss_batch_backproject: unsupported flags 0x%x",
                flags);
-  if (b->Ktot == 0) return VISS_OK;
+  if (b->Ktot == 0) return VISS_OK;  // no row: x unchanged (y may be NULL)
+  VISS_REQUIRE(y != nullptr, "viss_batch_backproject: y is NULL");
   hipStream_t s = as_stream(stream);
   VISS_REQUIRE(!use_graph || s != nullptr,
                "viss_batch_backproject: graph mode needs a non-default stream (capture is not allowed on stream 0)");
