@@ -26,7 +26,8 @@ def ops():
     return _ops
 
 
-@pytest.mark.parametrize("n_sat,nside,freq,sigma", [(4, 16, 1e6, 10e3), (10, 32, 10e6, 50e3), (16, 16, 10e6, 50e3)])
+@pytest.mark.parametrize("n_sat,nside,freq,sigma", [(2, 16, 1e6, 10e3), (4, 16, 1e6, 10e3), (10, 32, 10e6, 50e3),
+                                                    (16, 16, 10e6, 50e3)])
 def test_snapshot_backprojection(ops, n_sat, nside, freq, sigma):
     rng = np.random.default_rng(n_sat)
     th, ph, _, _ = oracle.pix2ang(nside, False)
@@ -49,7 +50,8 @@ def test_snapshot_backprojection(ops, n_sat, nside, freq, sigma):
         assert rel_l2(snap, ref) < 1e-5, flags
 
 
-@pytest.mark.parametrize("n_sat,nside,freq,sigma", [(3, 16, 1e6, 10e3), (8, 32, 10e6, 50e3), (10, 32, 10e6, 50e3),
+@pytest.mark.parametrize("n_sat,nside,freq,sigma", [(2, 16, 1e6, 10e3), (3, 16, 1e6, 10e3), (8, 32, 10e6, 50e3),
+                                                    (10, 32, 10e6, 50e3),
                                                     (12, 16, 1e6, 10e3), (16, 16, 10e6, 50e3)])
 def test_snapshot_forward(ops, n_sat, nside, freq, sigma):
     rng = np.random.default_rng(100 + n_sat)
@@ -80,3 +82,20 @@ def test_snapshot_backprojection_rejects_bad_input(ops):
         sky.backproject_snapshots(pos, 1.0, torch.zeros(2 * 136, dtype=torch.complex128).cuda())
     with pytest.raises(VissError):
         sky.backproject_snapshots(pos[:, :4], -1.0, torch.zeros(12, dtype=torch.complex128).cuda())
+
+
+def test_snapshot_counts_at_the_limits(ops):
+    """No snapshot: nothing to do (the arrays may be empty); 1 or 17 satellites: a status, not a launch."""
+    from viss_optimize_3d_amd._lib import VissError
+
+    sky = ops.Sky(torch.tensor([[1.0, 0.0], [0.0, 1.0], [0.0, 0.0]], dtype=torch.float64).cuda())
+    I = torch.ones(2, dtype=torch.float64).cuda()
+    V = sky.forward_snapshots(I, torch.zeros(0, 5, 3, dtype=torch.float64).cuda(), 1.0)
+    assert V.numel() == 0
+    x = sky.backproject_snapshots(torch.zeros(0, 5, 3, dtype=torch.float64).cuda(), 1.0,
+                                  torch.zeros(0, dtype=torch.complex128).cuda())
+    assert float(x.abs().max()) == 0.0
+    for n in (1, 17):
+        pos = torch.ones(3, n, 3, dtype=torch.float64).cuda()
+        with pytest.raises((VissError, ValueError)):
+            sky.forward_snapshots(I, pos, 1.0)
