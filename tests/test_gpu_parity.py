@@ -59,6 +59,29 @@ def test_pix2ang_parity(ops, nside, nest):
     assert np.all(d <= np.spacing(oth)), f"theta off by {np.max(d / np.spacing(oth))} ulp"
 
 
+@pytest.mark.parametrize("nest", [False, True])
+@pytest.mark.parametrize("nside", [4096, 8192])
+def test_pix2ang_largest_maps(ops, nside, nest):
+    """The reference's largest map (nside 4096, 201,326,592 pixels) and this library's largest
+    (nside 8192, 805,306,368 pixels, 19 GB of outputs): every pixel against the oracle, as
+    test_pix2ang_parity, compared in chunks of 2^26 pixels."""
+    th, ph, ring, iphi = ops.pix2ang(nside, nest=nest)
+    oth, oph, oring, oiphi = oracle.pix2ang(nside, nest)
+    P = 12 * nside * nside
+    step = 1 << 26
+    worst = 0.0
+    for a in range(0, P, step):
+        b = min(P, a + step)
+        np.testing.assert_array_equal(ring[a:b].cpu().numpy(), oring[a:b])
+        np.testing.assert_array_equal(iphi[a:b].cpu().numpy(), oiphi[a:b])
+        np.testing.assert_array_equal(ph[a:b].cpu().numpy(), oph[a:b])
+        d = np.abs(th[a:b].cpu().numpy() - oth[a:b]) / np.spacing(oth[a:b])
+        worst = max(worst, float(d.max()))
+    assert worst <= 1.0, f"theta off by {worst} ulp"
+    print(f"nside {nside} {'NEST' if nest else 'RING'}: {P} pixels, ring / iphi / phi exact, theta within "
+          f"{worst:g} ulp", flush=True)
+
+
 def test_pix2ang_rejects_bad_nside(ops):
     from viss_optimize_3d_amd._lib import VissError
 
