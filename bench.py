@@ -11,12 +11,22 @@ value = evaluated (visibility, pixel) pairs per second over forward + adjoint,
 where evaluated pairs are the pairs the occultation mask lets through (the
 exact count of m_kp = 1, from viss_sky_count_visible), counted once per pass.
 
-Printed extras: the all-FP64 GPU direct sum on the same work, relative L2 of
-the mixed result against it and against the FP64 CPU oracle on a sample, the
-FP64 CPU oracle's own throughput on that sample (OpenMP on the host cores),
-the reference's FP32 forward formulation on the host (throughput and its error
-against FP64), the notebook-style NumPy forward on the CPU-runnable case, and
-HBM rates of the pix2ang and weights passes.
+Printed extras (keys of the JSON line):
+  roofline      the dominant kernel (k_forward_quads) against its algorithmic flops
+                and the blended vector peak of that mix, with its committed VALU
+                issue fraction and HBM traffic, and the frozen per-pair rate beside it;
+  fp64_gpu      the all-FP64 GPU direct sum on the same work (its own per-pixel
+                kernels, and the mixed per-pixel kernels on the same layout), with the
+                mixed result's error against it;
+  cpu_baseline  the FP64 oracle on the host cores (OpenMP) on a sample of the rows,
+                the reference's FP32 forward formulation on the host (throughput and
+                error against FP64), the notebook-style NumPy forward (nside 16);
+  oracle_5pct   the mixed V and density-compensated image of a fixed 5% of the rows
+                against the FP64 oracle;
+  snapshot_*    the snapshot-structured kernels on the same rows;
+  hbm_passes    pix2ang (nside 256, 4096) and weights (1.8e6, 4e7 rows) HBM rates;
+  batched       5 frequencies x 64 groups at nside 128: the one-launch batch against
+                per-group launches, forward and density-compensated imaging.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload precision-nside256]
 """
