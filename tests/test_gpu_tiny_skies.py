@@ -4,7 +4,14 @@ pixels, plus a single antipodal pair.  Every layout the sky builder can pick
 for them (quads, pairs, per-pixel; partial tiles everywhere) runs the forward
 and the back-projection, mixed and all-FP64, masked and unmasked, against the
 FP64 oracle: relative L2 <= 1e-5 mixed, <= 1e-11 FP64.
+
+The back-projection of a sky of one or a few pixels is a sum over all rows
+whose terms have random phases, so it can cancel far below the size of its
+terms; its error is measured against the larger of the oracle image's norm
+and the norm such a sum has without cancellation, sqrt(P sum_k (wt_k |y_k|)^2).
 """
+import zlib
+
 import numpy as np
 import pytest
 
@@ -46,7 +53,7 @@ KINDS = ["healpix-1-ring", "healpix-1-nest", "healpix-2-ring", "healpix-2-nest",
 
 @pytest.mark.parametrize("kind", KINDS)
 def test_tiny_sky_against_oracle(ops, kind):
-    rng = np.random.default_rng(abs(hash(kind)) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(kind.encode()))
     lmn = tiny_sky(kind, rng)
     P = lmn.shape[1]
     I = rng.lognormal(0.0, 1.0, P)
@@ -58,6 +65,7 @@ def test_tiny_sky_against_oracle(ops, kind):
     l, m, n = lmn
     y = rng.standard_normal(K) + 1j * rng.standard_normal(K)
     wt = rng.uniform(0.5, 1.5, K)
+    incoherent = float(np.sqrt(P * np.sum((wt * np.abs(y)) ** 2)))
     worst = {}
     for mask in (True, False):
         flags = oracle.MASK if mask else 0
@@ -66,7 +74,8 @@ def test_tiny_sky_against_oracle(ops, kind):
         for fp64, tol in ((False, 1e-5), (True, 1e-11)):
             V = sky.forward(d(I), d(uvw), d(xa), d(xb), mask=mask, R_moon=R, fp64=fp64).cpu().numpy()
             x = sky.backproject(d(uvw), d(y), d(wt), d(xa), d(xb), mask=mask, R_moon=R, fp64=fp64).cpu().numpy()
-            rv, rx = rel_l2(V, Vo), rel_l2(x, xo)
+            rv = rel_l2(V, Vo)
+            rx = float(np.linalg.norm(x - xo) / max(np.linalg.norm(xo), incoherent))
             worst[(mask, fp64)] = max(rv, rx)
             assert rv <= tol and rx <= tol, (mask, fp64, rv, rx)
     print(f"{kind}: P={P} layout {sky.pairs()}; worst rel L2 {worst}", flush=True)
