@@ -7,11 +7,11 @@
 //   k_shell_hist : gs_k (FP32, bit-exact with the reference's expression) and
 //                  an LDS-privatised uint32 histogram of gs, flushed with one
 //                  global atomic per non-empty bin per block      [reads 4 B, writes 4 B per row]
-//   k_row_weights: the dcf_j table from the exact shell volumes and the counts,
-//                  built per block in LDS (block 0 writes it out; k_dcf_table
-//                  when the table exceeds LDS), then dg_k (one FP64 division),
-//                  wt_k = fmin(dcf[gs_k] dg_k, 1/8) with the table entry gathered
-//                  from LDS, gs_k recomputed from bll [reads 12 B, writes 8-16 B per row]
+//   k_dcf_table  : the dcf_j table from the exact shell volumes and the counts
+//   k_row_weights: the table copied into LDS per block (when it fits), then
+//                  dg_k (one FP64 division), wt_k = fmin(dcf[gs_k] dg_k, 1/8)
+//                  with the table entry gathered from LDS (else from L2), gs_k
+//                  recomputed from bll                [reads 12 B, writes 8-16 B per row]
 // All HBM-bound; the sort is dropped (its order is never used).
 #include <algorithm>
 #include <cstdint>
@@ -29,30 +29,53 @@ __device__ inline int32_t shell_index(float bll) {
   return (int32_t)c;
 }
 
-__device__ inline void count_shell(int32_t g, int32_t nr, uint32_t* hist, uint32_t* mb, int use_lds,
-                                   unsigned long long& bad) {
-  if (g >= 1 && g <= nr) {
-    if (use_lds)
-      atomicAdd(&hist[g - 1], 1u);
-    else
-      atomicAdd(&mb[g - 1], 1u);
-  } else if (g != 0) {
-    ++bad;
-  }
+// Shell histogram in LDS: rows outside [1, nr] add into one of 32 spare
+// slots past the table (one per LDS bank, chosen by lane, so that inputs with
+// many such rows do not serialise on one address), so that every row takes
+// the same branch-free add; bad counts those rows other than shell 0 (the
+// reference's shell index 0 is in range but uncounted).
+constexpr int kHistSpare = 32;
+
+__device__ __forceinline__ void count_lds(int32_t g, int32_t nr, uint32_t* hist, uint32_t& bad) {
+  const bool in = (uint32_t)g - 1u < (uint32_t)nr;  // g in [1, nr] (unsigned: no overflow at INT32_MIN)
+  bad += (uint32_t)(!in & (g != 0));
+  atomicAdd(&hist[in ? g - 1 : nr + (int)(threadIdx.x & (kHistSpare - 1))], 1u);
+}
+
+__device__ __forceinline__ void count_global(int32_t g, int32_t nr, uint32_t* mb, uint32_t& bad) {
+  const bool in = (uint32_t)g - 1u < (uint32_t)nr;  // g in [1, nr] (unsigned: no overflow at INT32_MIN)
+  bad += (uint32_t)(!in & (g != 0));
+  if (in) atomicAdd(&mb[g - 1], 1u);
+}
+
+__device__ __forceinline__ void count_bad(uint32_t bad, unsigned long long* n_bad) {
+  unsigned long long b = bad;
+  for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o, 64);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_bad, b);
 }
 
 // VEC: bll and gs are 16-B aligned and each thread takes 4 rows per step
 // (float4 in, int4 out): the LDS histogram allows only a few blocks per CU,
 // so the wider accesses are what keeps enough bytes in flight.
-template <bool VEC>
-__global__ __launch_bounds__(1024) void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t nr, int32_t* __restrict__ gs,
-                             uint32_t* __restrict__ mb, unsigned long long* __restrict__ n_bad, int use_lds) {
+// LDS: the block's histogram is in LDS (nr + kHistSpare counters) and is
+// flushed into mb with one global atomic per non-empty bin; otherwise every
+// row adds into mb directly.
+template <bool VEC, bool LDS>
+__global__ __launch_bounds__(1024) void k_shell_hist(const float* __restrict__ bll, int64_t K, int32_t nr,
+                                                     int32_t* __restrict__ gs, uint32_t* __restrict__ mb,
+                                                     unsigned long long* __restrict__ n_bad) {
   extern __shared__ uint32_t hist[];
-  if (use_lds) {
-    for (int j = threadIdx.x; j < nr; j += blockDim.x) hist[j] = 0u;
+  if (LDS) {
+    for (int j = threadIdx.x; j < nr + kHistSpare; j += blockDim.x) hist[j] = 0u;
     __syncthreads();
   }
-  unsigned long long bad = 0;
+  uint32_t bad = 0;
+  const auto count = [&](int32_t g) {
+    if (LDS)
+      count_lds(g, nr, hist, bad);
+    else
+      count_global(g, nr, mb, bad);
+  };
   const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
   int64_t k_tail = 0;
   if (VEC) {
@@ -62,10 +85,10 @@ __global__ __launch_bounds__(1024) void k_shell_hist(const float* __restrict__ b
     auto one = [&](const float4 b, int64_t q) {
       const int4 g = make_int4(shell_index(b.x), shell_index(b.y), shell_index(b.z), shell_index(b.w));
       g4[q] = g;
-      count_shell(g.x, nr, hist, mb, use_lds, bad);
-      count_shell(g.y, nr, hist, mb, use_lds, bad);
-      count_shell(g.z, nr, hist, mb, use_lds, bad);
-      count_shell(g.w, nr, hist, mb, use_lds, bad);
+      count(g.x);
+      count(g.y);
+      count(g.z);
+      count(g.w);
     };
     int64_t q = t0;
     for (; q + 7 * stride < K4; q += 8 * stride) {  // eight loads in flight before the atomics
@@ -81,13 +104,10 @@ __global__ __launch_bounds__(1024) void k_shell_hist(const float* __restrict__ b
   for (int64_t k = k_tail + t0; k < K; k += stride) {
     const int32_t g = shell_index(bll[k]);
     gs[k] = g;
-    count_shell(g, nr, hist, mb, use_lds, bad);
+    count(g);
   }
-  if (n_bad) {
-    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(n_bad, bad);
-  }
-  if (use_lds) {
+  if (n_bad) count_bad(bad, n_bad);
+  if (LDS) {
     __syncthreads();
     for (int j = threadIdx.x; j < nr; j += blockDim.x) {
       const uint32_t c = hist[j];
@@ -146,62 +166,95 @@ __device__ inline double row_wt(int32_t s, int32_t nr, double dg, const double* 
 
 // VEC: bll 8-B, w / wt / dg 16-B aligned; rows go in pairs (one 8-B load of
 // bll, one 16-B load of w, one 16-B store of wt and of dg), so the lanes of a
-// wavefront load and store contiguous 512-B / 1-KB spans, and each thread has
-// four pairs in flight per step.  The shell index is recomputed from bll (the
-// same expression as k_shell_hist, so the same gs) instead of read back.
-// LDS_TAB: the block first builds the dcf table in LDS from the shell counts
-// (block 0 also writes it to dcf_g) and the rows gather from there.  Rows hit
-// shells at random, so gathering from a global table would cost a cache-line
-// request per row.  Otherwise the table is in dcf_g already (k_dcf_table).
+// wavefront load and store contiguous 512-B / 1-KB spans.  The loop is
+// software-pipelined: the next two pairs' loads are issued before the current
+// two are computed, so the loads stay in flight during the FP64 division and
+// square root (a block holding the 128-KB table of 16,000 shells leaves room
+// for one block, four wavefronts per SIMD, per CU).  The shell index is
+// recomputed from bll (the same expression as k_shell_hist, so the same gs)
+// instead of read back.
+// LDS_TAB: the block first copies the dcf table (written by k_dcf_table) into
+// LDS, 16 loads in flight per thread, and the rows gather from there.  Rows hit
+// shells at random, so gathering from the global table costs a cache-line
+// request per row.
 template <bool VEC, bool LDS_TAB>
 __global__ __launch_bounds__(1024) void k_row_weights(const float* __restrict__ bll, const double* __restrict__ w,
-                                                      int64_t K, int32_t nr, const uint32_t* __restrict__ mb,
-                                                      int shift, double* __restrict__ dcf_g,
+                                                      int64_t K, int32_t nr, const double* __restrict__ dcf_g,
                                                       double* __restrict__ dg_out, double* __restrict__ wt) {
   extern __shared__ double tab_s[];
   const double* __restrict__ dcf = dcf_g;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  constexpr int F = 2;  // pairs per pipeline stage
+  const int64_t K2 = VEC ? K / 2 : 0;
+  const float2* __restrict__ b2 = reinterpret_cast<const float2*>(bll);
+  const double2* __restrict__ w2 = reinterpret_cast<const double2*>(w);
+  int64_t q = t0;
+  float2 b[F];
+  double2 wk[F];
+  if (VEC && q + (F - 1) * stride < K2) {  // the first stage's loads go out before the table copy
+#pragma unroll
+    for (int i = 0; i < F; ++i) {
+      b[i] = b2[q + i * stride];
+      wk[i] = w2[q + i * stride];
+    }
+  }
   if (LDS_TAB) {
-    for (int j = threadIdx.x; j <= nr; j += blockDim.x) {
-      const double d = dcf_value(j, mb[dcf_slot(j, nr, shift)], nr, shift);
-      tab_s[j] = d;
-      if (blockIdx.x == 0) dcf_g[j] = d;
+    constexpr int B = 16;
+    for (int j0 = 0; j0 <= nr; j0 += B * (int)blockDim.x) {
+      double d[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int j = j0 + i * (int)blockDim.x + (int)threadIdx.x;
+        d[i] = dcf_g[j <= nr ? j : 0];
+      }
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int j = j0 + i * (int)blockDim.x + (int)threadIdx.x;
+        if (j <= nr) tab_s[j] = d[i];
+      }
     }
     __syncthreads();
     dcf = tab_s;
   }
-  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
   int64_t k_tail = 0;
   if (VEC) {
-    const int64_t K2 = K / 2;
-    const float2* __restrict__ b2 = reinterpret_cast<const float2*>(bll);
-    const double2* __restrict__ w2 = reinterpret_cast<const double2*>(w);
     double2* __restrict__ t2 = reinterpret_cast<double2*>(wt);
     double2* __restrict__ g2 = reinterpret_cast<double2*>(dg_out);
-    const auto pair = [&](const float2 b, const double2 wk, int64_t q) {
-      const double d0 = row_dg(b.x, wk.x), d1 = row_dg(b.y, wk.y);
-      t2[q] = make_double2(row_wt(shell_index(b.x), nr, d0, dcf), row_wt(shell_index(b.y), nr, d1, dcf));
-      if (dg_out) g2[q] = make_double2(d0, d1);
+    const auto pair = [&](const float2 bb, const double2 ww, int64_t qq) {
+      const double d0 = row_dg(bb.x, ww.x), d1 = row_dg(bb.y, ww.y);
+      t2[qq] = make_double2(row_wt(shell_index(bb.x), nr, d0, dcf), row_wt(shell_index(bb.y), nr, d1, dcf));
+      if (dg_out) g2[qq] = make_double2(d0, d1);
     };
-    int64_t q = t0;
-    for (; q + 3 * stride < K2; q += 4 * stride) {  // four pairs in flight
-      float2 b[4];
-      double2 wk[4];
+    for (; q + (F - 1) * stride < K2; q += F * stride) {
+      const int64_t qn = q + F * stride;
+      const bool more = qn + (F - 1) * stride < K2;
+      float2 nb[F];
+      double2 nw[F];
+      if (more) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        b[i] = b2[q + i * stride];
-        wk[i] = w2[q + i * stride];
+        for (int i = 0; i < F; ++i) {
+          nb[i] = b2[qn + i * stride];
+          nw[i] = w2[qn + i * stride];
+        }
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) pair(b[i], wk[i], q + i * stride);
+      for (int i = 0; i < F; ++i) pair(b[i], wk[i], q + i * stride);
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < F; ++i) {
+          b[i] = nb[i];
+          wk[i] = nw[i];
+        }
+      }
     }
     for (; q < K2; q += stride) pair(b2[q], w2[q], q);
     k_tail = K2 * 2;
   }
   for (int64_t k = k_tail + t0; k < K; k += stride) {
-    const float b = bll[k];
-    const double dgk = row_dg(b, w[k]);
+    const float bk = bll[k];
+    const double dgk = row_dg(bk, w[k]);
     if (dg_out) dg_out[k] = dgk;
-    wt[k] = row_wt(shell_index(b), nr, dgk, dcf);
+    wt[k] = row_wt(shell_index(bk), nr, dgk, dcf);
   }
 }
 
@@ -214,18 +267,15 @@ __global__ __launch_bounds__(1024) void k_batch_shell_hist(const float* __restri
                                                            unsigned long long* __restrict__ n_bad) {
   extern __shared__ uint32_t hist[];
   const RowChunk c = ch[blockIdx.x];
-  for (int j = threadIdx.x; j < nr; j += blockDim.x) hist[j] = 0u;
+  for (int j = threadIdx.x; j < nr + kHistSpare; j += blockDim.x) hist[j] = 0u;
   __syncthreads();
-  unsigned long long bad = 0;
+  uint32_t bad = 0;
   for (int64_t k = c.k0 + threadIdx.x; k < c.k1; k += blockDim.x) {
     const int32_t g = shell_index(bll[k]);
     gs[k] = g;
-    count_shell(g, nr, hist, nullptr, 1, bad);
+    count_lds(g, nr, hist, bad);
   }
-  if (n_bad) {
-    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(n_bad, bad);
-  }
+  if (n_bad) count_bad(bad, n_bad);
   __syncthreads();
   uint32_t* __restrict__ mbg = mb + c.g * (int64_t)nr;
   for (int j = threadIdx.x; j < nr; j += blockDim.x) {
@@ -277,7 +327,7 @@ int batch_weights(const float* bll, const double* w, const RowChunk* wch, int64_
     return VISS_ERR_SIZE;
   }
   VISS_REQUIRE(nwch >= 1 && nwch <= INT32_MAX, "viss_batch_weights: %lld weight chunks", (long long)nwch);
-  k_batch_shell_hist<<<(unsigned)nwch, 1024, (size_t)nr * sizeof(uint32_t), s>>>(
+  k_batch_shell_hist<<<(unsigned)nwch, 1024, ((size_t)nr + kHistSpare) * sizeof(uint32_t), s>>>(
       bll, wch, nr, gs, mb, (unsigned long long*)n_out_of_range);
   VISS_LAUNCHED(k_batch_shell_hist);
   const int shift = (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0;
@@ -304,14 +354,16 @@ extern "C" int viss_weights(const float* bll, const double* w, int64_t K, int32_
   VISS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   VISS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   if (K > 0) {
-    const size_t lds = (size_t)nr * sizeof(uint32_t);
-    const int use_lds = lds <= (size_t)lds_max ? 1 : 0;
-    // as many resident blocks as the CUs' LDS holds (2..8 per CU): each block
-    // flushes its nr counters once, so no more than that
+    const size_t lds = ((size_t)nr + kHistSpare) * sizeof(uint32_t);
+    const bool use_lds = lds <= (size_t)lds_max;
+    // half as many blocks per CU as its LDS holds tables (1..8): every block
+    // ends by flushing its nr counters with global atomics on the same
+    // addresses, and with two 64-KB tables per CU that flush took 12 of 75 us
+    // at 4e7 rows (one per CU: 65 us)
     int lds_cu = 0;
     VISS_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    int64_t per_cu = use_lds && lds > 0 ? (int64_t)lds_cu / (int64_t)lds : 8;
-    per_cu = per_cu < 2 ? 2 : (per_cu > 8 ? 8 : per_cu);
+    int64_t per_cu = use_lds && lds > 0 ? (int64_t)lds_cu / (2 * (int64_t)lds) : 8;
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
     // 1024-thread blocks: the per-CU block count is small (the LDS table),
     // so large blocks keep enough loads in flight
     const int hb = use_lds ? 1024 : 256;
@@ -320,44 +372,46 @@ extern "C" int viss_weights(const float* bll, const double* w, int64_t K, int32_
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
     const bool vec = ((uintptr_t)bll % 16 == 0) && ((uintptr_t)gs % 16 == 0);
-    if (vec)
-      k_shell_hist<true><<<(unsigned)blocks, hb, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
-                                                                        (unsigned long long*)n_out_of_range, use_lds);
-    else
-      k_shell_hist<false><<<(unsigned)blocks, hb, use_lds ? lds : 0, s>>>(bll, K, nr, gs, mb,
-                                                                         (unsigned long long*)n_out_of_range, use_lds);
+    unsigned long long* nb = (unsigned long long*)n_out_of_range;
+#define VISS_SH(V, L) k_shell_hist<V, L><<<(unsigned)blocks, hb, L ? lds : 0, s>>>(bll, K, nr, gs, mb, nb)
+    if (vec) {
+      if (use_lds) VISS_SH(true, true); else VISS_SH(true, false);
+    } else {
+      if (use_lds) VISS_SH(false, true); else VISS_SH(false, false);
+    }
+#undef VISS_SH
     VISS_LAUNCHED(k_shell_hist);
   }
   const int shift = (flags & VISS_DCF_BLOCKAGE_SHIFT) ? 1 : 0;
   const size_t tab = ((size_t)nr + 1) * sizeof(double);
-  // the row kernel builds the table in LDS when it fits and the rows are many
-  // enough to pay for one table per block; otherwise a launch of its own writes
-  // it (also for K = 0: the table of an empty histogram) and the rows gather it
-  // from L2.  Measured with 1024-thread blocks: 4e7 rows over 16,012 shells,
-  // 171 us from LDS against 330 us from L2; 1.8e6 rows over 11,082 shells (162
-  // rows per shell), 13.5 us from L2 against 15.6 us from LDS.
-  const bool lds_tab = K > 0 && tab <= (size_t)lds_max && K >= 512 * ((int64_t)nr + 1);
-  if (!lds_tab) {
+  // the table (also for K = 0: the table of an empty histogram); the row
+  // kernel copies it into LDS when it fits and the rows are many enough to pay
+  // for one copy per block, else the rows gather it from L2.  Measured with
+  // 1024-thread blocks: 4e7 rows over 16,012 shells, 171 us with a table in
+  // LDS against 330 us from L2; 1.8e6 rows over 11,082 shells (162 rows per
+  // shell), 13.5 us from L2 against 15.6 us from LDS.
+  {
     int64_t g = ((int64_t)nr + 1 + 255) / 256;
     if (g > 4096) g = 4096;
     k_dcf_table<<<(unsigned)g, 256, 0, s>>>(mb, nr, shift, dcf);
     VISS_LAUNCHED(k_dcf_table);
   }
+  const bool lds_tab = K > 0 && tab <= (size_t)lds_max && K >= 512 * ((int64_t)nr + 1);
   if (K > 0) {  // row weights, gathering the table
     const bool vec = ((uintptr_t)bll % 8 == 0) && ((uintptr_t)w % 16 == 0) && ((uintptr_t)wt % 16 == 0) &&
                      (dg == nullptr || (uintptr_t)dg % 16 == 0);
-    // LDS path: 1024-thread blocks, as many per CU as the table copies allow
-    // (each block builds the table once); else 256-thread blocks
+    // LDS path: 1024-thread blocks, as many per CU as the table copies allow;
+    // else 256-thread blocks
     const int block = lds_tab ? 1024 : 256;
     int lds_cu = 0;
     VISS_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     int64_t per_cu = lds_tab ? std::max<int64_t>(1, std::min<int64_t>(2, (int64_t)lds_cu / (int64_t)tab)) : 64;
-    int64_t g = ((vec ? K / 8 : K) + block - 1) / block;  // four row pairs per thread and step
+    int64_t g = ((vec ? K / 4 : K) + block - 1) / block;  // two row pairs per thread and stage
     const int64_t cap = (int64_t)(ncu > 0 ? ncu : 256) * per_cu;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
 #define VISS_RW(V, L) \
-  k_row_weights<V, L><<<(unsigned)g, block, L ? tab : 0, s>>>(bll, w, K, nr, mb, shift, dcf, dg, wt)
+  k_row_weights<V, L><<<(unsigned)g, block, L ? tab : 0, s>>>(bll, w, K, nr, dcf, dg, wt)
     if (vec) {
       if (lds_tab) VISS_RW(true, true); else VISS_RW(true, false);
     } else {
