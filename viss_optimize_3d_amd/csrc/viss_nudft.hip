@@ -428,13 +428,55 @@ __device__ __forceinline__ void quad_bp_rows(uint64_t msk, const VisHot* __restr
   }
 }
 
+// quad_bp_rows for a step whose 64 rows all take modes (MZ, MM), neither
+// tested per pixel (the common case away from the occultation edges): the
+// rows go in order, eight at a time, so their LDS reads and record loads take
+// compile-time offsets, where the mask loop spends a vector add per row on the
+// lane's LDS address (71.4 -> 70.4 ms with this for (FF, FF) alone, 67.3 ms
+// for every such category, on the precision workload).
+template <int MZ, int MM>
+__device__ __forceinline__ void quad_bp_rows_all(const VisHot* __restrict__ hot, const float4* __restrict__ rot,
+                                                 int64_t kb, double qx, double qy, QuadBpAcc& acc) {
+  static_assert(MZ < kRingTF && MM < kRingTF, "rows without a per-pixel test only");
+  const VisHot* __restrict__ hb = hot + kb;
+  for (int j0 = 0; j0 < 64; j0 += 8) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const VisHot h = hb[j0 + i];
+      const float4 r = rot[j0 + i];
+      float sn, cs;
+      sincos_rev(rev_fp32(fma(h.u, qx, h.v * qy)), sn, cs);
+      ring_bp<MZ>(r.x, r.y, cs, sn, false, false, acc.Pz, acc.Qz, acc.Az, acc.Bz);
+      ring_bp<MM>(r.z, r.w, cs, sn, false, false, acc.Pm, acc.Qm, acc.Cm, acc.Dm);
+    }
+  }
+}
+
+// The rows of one (ring z, ring -z) mode category.
+template <int MZ, int MM>
+__device__ __forceinline__ void quad_bp_cat(uint64_t msk, const VisHot* __restrict__ hot,
+                                            const VisMask* __restrict__ vm, const float4* __restrict__ rot,
+                                            int64_t kb, double qx, double qy, double zl, QuadBpAcc& acc) {
+  if constexpr (MZ < kRingTF && MM < kRingTF) {
+    if (msk == ~0ull) {
+      quad_bp_rows_all<MZ, MM>(hot, rot, kb, qx, qy, acc);
+      return;
+    }
+  }
+  quad_bp_rows<MZ, MM>(msk, hot, vm, rot, kb, qx, qy, zl, acc);
+}
+
 // One wavefront per patch (4 tiles of 16 quads, one per ring of a band), one
 // quad per lane (lanes 16g..16g+15 take tile patch[g]).  Per 64-row step,
 // lane j loads row j, writes its y rotated by the four rings' factors to the
 // wavefront's LDS slice and classifies the row against the patch's four
 // sides; the rows then run in one loop per (ring z, ring -z) mode pair.
+// Held to 7 wavefronts per SIMD (at most 72 VGPRs): the unrolled whole-step
+// row loops need 66 VGPRs, and squeezed into 64 (8 wavefronts) they spill 12
+// bytes per lane to scratch and ran 0.6 ms slower; 6 wavefronts 0.2-0.4 ms
+// slower.
 template <bool MASK, bool CHUNKED>
-__global__ __launch_bounds__(256) void k_backproject_quads(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_backproject_quads(
     const QuadPack* __restrict__ qp, const int4* __restrict__ qi, const double* __restrict__ qz,
     const int4* __restrict__ patch, const float4* __restrict__ pcone, int64_t npatch, const VisHot* __restrict__ hot,
     const VisMask* __restrict__ vm, const VisCls* __restrict__ vc, int64_t K, int64_t k_per_chunk, double herm,
@@ -509,7 +551,7 @@ __global__ __launch_bounds__(256) void k_backproject_quads(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float4* __restrict__ rot = rot_w + g * 64;
-#define VISS_QB(A, B) quad_bp_rows<A, B>(bz[A] & bm[B], hot, vm, rot, kb, qx, qy, zl, acc);
+#define VISS_QB(A, B) quad_bp_cat<A, B>(bz[A] & bm[B], hot, vm, rot, kb, qx, qy, zl, acc);
 #define VISS_QB_ROW(A)                                                                                   \
   if (bz[A]) {                                                                                           \
     VISS_QB(A, kRingNone) VISS_QB(A, kRingFF) VISS_QB(A, kRingFN) VISS_QB(A, kRingNF) VISS_QB(A, kRingTF) \
