@@ -379,18 +379,32 @@ __device__ __forceinline__ void ring_step(float w1, float w2, float sum, float d
 }
 
 template <int MZ, int MM>
+__device__ __forceinline__ void quad_one(const QuadPack& q, double u, double v, const VisMask& mk, const QuadOcc& zz,
+                                         float& Rz, float& Jz, float& Rm, float& Jm) {
+  float sn, cs;
+  sincos_rev(rev_fp32(fma(u, q.x, v * q.y)), sn, cs);
+  bool va, vb, vc, vd;
+  quad_visible<MZ, MM>(q.x, q.y, mk, zz, va, vb, vc, vd);
+  ring_step<MZ>(q.Ia, q.Ib, q.Az, q.Dz, va, vb, cs, sn, Rz, Jz);
+  ring_step<MM>(q.Ic, q.Id, q.Am, q.Dm, vc, vd, cs, sn, Rm, Jm);
+}
+
+// The 16 quads of a tile, each quad's record loaded (scalar loads) one quad
+// ahead of its use, so that the loads of quad j + 1 are in flight while quad
+// j computes: 83.8 -> 82.5 ms on the precision workload against four quads
+// loaded and then computed at a time (two ahead 83.2 ms, four ahead 85.2 ms
+// with 104 SGPRs and one wavefront per SIMD fewer).
+template <int MZ, int MM>
 __device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ tq, double u, double v,
                                                   const VisMask& mk, const QuadOcc& zz, float& Rz, float& Jz,
                                                   float& Rm, float& Jm) {
-#pragma unroll 4
+  QuadPack cur = tq[0];
+#pragma unroll
   for (int j = 0; j < kQuadTile; ++j) {
-    const QuadPack q = tq[j];
-    float sn, cs;
-    sincos_rev(rev_fp32(fma(u, q.x, v * q.y)), sn, cs);
-    bool va, vb, vc, vd;
-    quad_visible<MZ, MM>(q.x, q.y, mk, zz, va, vb, vc, vd);
-    ring_step<MZ>(q.Ia, q.Ib, q.Az, q.Dz, va, vb, cs, sn, Rz, Jz);
-    ring_step<MM>(q.Ic, q.Id, q.Am, q.Dm, vc, vd, cs, sn, Rm, Jm);
+    QuadPack nxt = cur;
+    if (j + 1 < kQuadTile) nxt = tq[j + 1];
+    quad_one<MZ, MM>(cur, u, v, mk, zz, Rz, Jz, Rm, Jm);
+    cur = nxt;
   }
 }
 
