@@ -253,6 +253,24 @@ def test_weights_shell_table_beyond_lds(ops):
     check_weights(ops.weights(cuda(bll), cuda(w), nr), bll, w, nr)
 
 
+def test_weights_mostly_out_of_range(ops):
+    """Most rows outside the shell table (shell 0, past nr, NaN bll): the
+    histogram adds them into spare counters past its LDS table, all on a few
+    addresses; counts, the out-of-range tally, dcf and wt as the oracle's.
+    2e6 rows over 2,000 shells take the LDS paths of both passes."""
+    rng = np.random.default_rng(23)
+    K, nr = 2_000_000, 2_000
+    kind = rng.integers(0, 10, K)
+    bll = rng.uniform(0.3, nr / 2.0, K).astype(np.float32)  # in range
+    bll[kind < 4] = 0.0                                       # shell 0: not counted, not out of range
+    bll[(kind >= 4) & (kind < 7)] = np.float32(nr)            # past shell nr
+    bll[kind == 7] = np.nan
+    w = rng.uniform(-1, 1, K) * np.nan_to_num(bll) * 0.9
+    out = ops.weights(cuda(bll), cuda(w), nr)
+    check_weights(out, bll, w, nr)
+    assert int(out["n_out_of_range"].item()) == int(np.sum((kind >= 4) & (kind < 8)))
+
+
 def test_weights_unaligned_and_ragged(ops):
     """Row counts that are not multiples of the 4-row / 2-row vector steps
     (the tail loops), and views one element into their buffers (the scalar
