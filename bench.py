@@ -96,6 +96,9 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched one-launch comparison")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched comparison (profiling runs of that workload)")
+    ap.add_argument("--batched-forward-only", action="store_true",
+                    help="build the batched workload and run two batched forward launches, nothing else "
+                         "(counter runs of the batch kernel: the per-group comparison serialises ~1e4 launches)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the HBM-traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     ap.add_argument("--summarize-valu", metavar="PMC_DIR",
@@ -233,6 +236,9 @@ def main():
         del src, dst
     if args.batched_only:
         print(json.dumps({"batched": batched(ops, geo, dev)}), flush=True)
+        return
+    if args.batched_forward_only:
+        batched_forward_launches(ops, geo, dev)
         return
     wl = geo.WORKLOADS[args.workload]
     log(f"[bench] generating {wl.name}: nside={wl.nside} K={wl.n_vis}")
@@ -568,6 +574,24 @@ def cpu_baseline(args, inp, lmn, I, V_mixed, x_mixed, sky, uvw, xa, xb, R):
                       f"density-compensated back-projection (OpenMP), {tf + ta:.1f} s",
             "reference_fp32_forward": ref32, "numpy_notebook_forward": numpy1}
     return base, acc
+
+
+def batched_forward_launches(ops, geo, dev, n=2):
+    """The batched workload's work-table forward, n launches (default table), for counter runs."""
+    import torch
+
+    B, uvw, xa, xb, offs, gf = geo.batched_workload()
+    th, ph = ops.pix2ang(128, nest=False, with_ring=False)
+    lmn, I = ops.pixel_dirs(th, ph, torch.from_numpy(B).to(dev))
+    sky = ops.Sky(lmn)
+    U, A, Bx = (torch.from_numpy(a).to(dev) for a in (uvw, xa, xb))
+    V = torch.zeros(U.shape[1], dtype=torch.complex128, device=dev)
+    batch = ops.Batch(sky, offs, U, A, Bx, mask=True, R_moon=geo.R_MOON)
+    log(f"[bench] batched forward only: {len(offs) - 1} groups, {U.shape[1]} rows")
+    for i in range(n):
+        batch.forward(I, out=V)
+        torch.cuda.synchronize()
+        log(f"[bench] batched forward launch {i + 1}/{n} done")
 
 
 def batched(ops, geo, dev):
