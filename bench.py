@@ -70,12 +70,17 @@ PAIR_T_PEAK = PAIR_FP32 / PEAK_FP32 + PAIR_FP64 / PEAK_FP64  # s per pair at pea
 # The occultation classification and per-pixel tests are not counted: they decide
 # which of these units run, and their cost shows in the VALU issue fraction.
 QUAD_WORK = {"row_quads": (4, 20), "row_rings": (0, 4), "row_tiles": (2, 32), "row_rest_pairs": (6, 24)}
+# Skies whose tiles take the recurrent phase (HEALPix rings: equal azimuth steps):
+# A_{j+1} = 2 cos(d) A_j - A_{j-1}, one FMA (2 FP64) per (row, quad) from a tile's
+# third quad on; the first two keep mul + FMA, charged per (row, tile) (+2 FP64).
+QUAD_WORK_REC = {"row_quads": (3, 20), "row_rings": (0, 4), "row_tiles": (4, 32), "row_rest_pairs": (6, 24)}
 
 
-def quad_flops(work):
+def quad_flops(work, rec=False):
     """(FP64 flops, FP32 flops) of one pass of the quad kernels over counted work."""
-    f64 = sum(work[k] * QUAD_WORK[k][0] for k in QUAD_WORK)
-    f32 = sum(work[k] * QUAD_WORK[k][1] for k in QUAD_WORK)
+    table = QUAD_WORK_REC if rec else QUAD_WORK
+    f64 = sum(work[k] * table[k][0] for k in table)
+    f32 = sum(work[k] * table[k][1] for k in table)
     return f64, f32
 
 
@@ -131,10 +136,10 @@ def summarize_pmc(fetch_dir, write_dir, calib_mib=1024):
     out = [f"calibration: {calib_mib} MiB copy reads FETCH_SIZE={max(copies_f):.1f} KiB, WRITE_SIZE={max(copies_w):.1f} KiB",
            f"scale factors: fetch x{ff:.4f}, write x{fw:.4f}"]
     # back-projection kernels: <MASK, CHUNKED> (<MASK, F64, CHUNKED> per pixel); both row-chunk variants
-    keys = ("k_forward_quads<true>", "k_backproject_quads<true,", "k_forward_pairs<true>",
+    keys = ("k_forward_quads<true,", "k_backproject_quads<true,", "k_forward_pairs<true>",
             "k_backproject_pairs<true,", "k_forward<true, false>",
             "k_backproject<true, false,", "k_forward_snap_pairs<true, 10>", "k_backproject_snap_pairs<true, 10>",
-            "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_quads<true>",
+            "k_forward_snap<true, 10>", "k_backproject_snap<true, 10>", "k_forward_batch_quads<true,",
             "k_forward_batch_pairs<true>", "k_forward_batch<true, false>",
             "k_pix2ang", "k_pixel_dirs",
             "k_bp_prologue", "k_shell_hist", "k_row_weights")
@@ -320,14 +325,17 @@ def main():
     if quads:
         work = sky.count_work(xa, xb, R_moon=R)
         assert work["visible_pairs"] == visible
-        f64, f32 = quad_flops(work)
+        rec = layout["quad_recurrence"]
+        f64, f32 = quad_flops(work, rec)
         t_peak = f32 / PEAK_FP32 + f64 / PEAK_FP64
         out["roofline"] = {
             "bound": "valu", "kernel": dom, "achieved": (f32 + f64) / (dom_ms * 1e-3) / 1e12,
             "peak": (f32 + f64) / t_peak / 1e12, "unit": "TFLOP/s", "frac": t_peak / (dom_ms * 1e-3),
             "traffic": committed_traffic(dom, wl.name), "kernel_ms": dom_ms,
             "flops_per_launch": {"fp64": f64, "fp32": f32}, "work_per_launch": work,
-            "flops_per_unit": {k: {"fp64": a, "fp32": b} for k, (a, b) in QUAD_WORK.items()},
+            "flops_per_unit": {k: {"fp64": a, "fp32": b}
+                               for k, (a, b) in (QUAD_WORK_REC if rec else QUAD_WORK).items()},
+            "recurrent_phase": rec,
             "note": ("algorithmic flops of the quad kernels that run (units counted exactly by "
                      "viss_sky_count_work); peak = the blended FP32 / FP64 vector peak for that mix; "
                      "frac = time at peak / kernel time"),

@@ -166,8 +166,11 @@ int64_t viss_sky_npix(const viss_sky* sky);
  * nullable); *uses_pairs = 1 when the mixed per-row kernels use the pairs. */
 int viss_sky_pairs(const viss_sky* sky, int64_t* npair, int64_t* nsingle, int* uses_pairs);
 /* Equatorial-mirror quads found at creation, their 16-quad tiles and
- * whether the mixed per-row forward uses them (host pointers, nullable).
- * Layout query only: no counterpart in the reference. */
+ * whether the mixed per-row forward uses them (host pointers, nullable):
+ * *uses_quads = 0 no, 1 yes, 2 yes with the recurrent phase (every tile's
+ * quads at equal azimuth steps on one circle, as on a HEALPix ring: one FP64
+ * FMA a quad for u x + v y).  Layout query only: no counterpart in the
+ * reference. */
 int viss_sky_quads(const viss_sky* sky, int64_t* nquad, int64_t* nquad_tiles, int* uses_quads);
 
 int viss_sky_forward(viss_sky* sky, const double* I, const double* uvw, const double* xyz_a, const double* xyz_b,

@@ -149,7 +149,9 @@ def ring_sky(rng, nring=20, per_ring=48):
 
 def test_quad_counts(ops):
     """HEALPix: every pair off the equator ring is in a quad; the tiles are
-    full enough for the quad forward from nside 8 on."""
+    full enough for the quad forward from nside 8 on, and their quads sit at
+    equal azimuth steps, so the forward takes the recurrent phase.  Quads at
+    random azimuths keep the direct phase."""
     rng = np.random.default_rng(4)
     for nside, nest in ((16, False), (8, True), (64, False)):
         th, ph, _, _ = oracle.pix2ang(nside, nest)
@@ -158,9 +160,22 @@ def test_quad_counts(ops):
         print(f"healpix nside={nside} nest={nest}: {got}", flush=True)
         P = lmn.shape[1]
         assert got["pairs"] == P // 2 and got["quads"] == (P // 2 - 2 * nside) // 2 and got["uses_quads"]
+        assert got["quad_recurrence"]
+    # one quad of a HEALPix sky moved off its ring's azimuth grid by 8e-13 in x
+    # (mirror images kept exact): still a quad, but the sky keeps the direct phase
+    l, m, n = lmn = lmn.copy()
+    i = int(np.argmax(n < 0.6)) + 2  # third pixel of a ring of 256: the third quad of its tile
+    for s in (1.0, -1.0):
+        j = np.where((np.abs(l - s * l[i]) < 1e-9) & (np.abs(m - s * m[i]) < 1e-9) & (np.abs(np.abs(n) - n[i]) < 1e-9))[0]
+        assert len(j) == 2
+        l[j] += s * 8e-13
+    got = ops.Sky(torch.from_numpy(lmn).cuda()).pairs()
+    print(f"healpix nside=64, one quad moved: {got}", flush=True)
+    assert got["uses_quads"] and got["quads"] == (P // 2 - 2 * 64) // 2 and not got["quad_recurrence"]
     got = ops.Sky(torch.from_numpy(ring_sky(rng)).cuda()).pairs()
     print(f"rings: {got}", flush=True)
     assert got["quads"] == 20 * 48 and got["quad_tiles"] == 20 * 3 and got["uses_quads"]
+    assert not got["quad_recurrence"]
     got = ops.Sky(torch.from_numpy(sky_of("exact-pm", rng)).cuda()).pairs()
     assert got["quads"] == 0 and not got["uses_quads"]
     # quads in tiles under 2/3 full (3 per ring): the pair kernels run instead

@@ -239,9 +239,10 @@ __global__ __launch_bounds__(256) void k_forward_batch_pairs(const PairPack* __r
 // gives the kernel over 100 VGPRs (4 wavefronts per SIMD).  Held to 7
 // wavefronts it fits in 66 without spilling (8 would spill to scratch): at
 // 4.48e6 rows x 196,608 pixels, 102.9 ms per pass against 128 ms at 4.
-template <bool MASK>
+template <bool MASK, bool REC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_forward_batch_quads(const QuadPack* __restrict__ qp,
                                                              const double* __restrict__ qz,
+                                                             const double* __restrict__ qc2,
                                                              const float4* __restrict__ qcone, int64_t nqtile,
                                                              const PairPack* __restrict__ rp,
                                                              const float4* __restrict__ rcone,
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
           quad_tile_modes(qcone, t, __builtin_amdgcn_readlane(code, (int)(j - jb)), r, actm, mz, mm);
           if (mz == kRingNone && mm == kRingNone) continue;
         }
-        quad_tile<MASK>(qp, qz, t, mz, mm, r, qs, Sr, Si);
+        quad_tile<MASK, REC>(qp, qz, qc2, t, mz, mm, r, qs, Sr, Si);
         qs.flush(Sr, Si);  // per tile: the sums do not depend on how the table cuts the list
       } else {
         forward_pair_tiles<MASK>(rp, rcone, nullptr, t - nqtile, t - nqtile + 1, r, Sr, Si);
@@ -416,14 +417,11 @@ static int launch_batch(viss_batch* b, const WorkItem* items, int64_t n_items, d
   const int track = (b->flags & VISS_PHASE_TRACK) ? 1 : 0;
   const bool mask = b->flags & VISS_MASK, f64 = b->flags & VISS_PREC_FP64;
   if (b->quads) {
-    if (mask)
-      k_forward_batch_quads<true><<<grid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, sky->rpack,
-                                                       sky->rcone, b->tiles, b->ntile, items, n_items, b->uvw, b->xa,
-                                                       b->xb, b->Ktot, b->R, b->orb, track, V);
-    else
-      k_forward_batch_quads<false><<<grid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, sky->rpack,
-                                                        sky->rcone, b->tiles, b->ntile, items, n_items, b->uvw, b->xa,
-                                                        b->xb, b->Ktot, b->R, b->orb, track, V);
+    auto* kern = mask ? (sky->qrec ? k_forward_batch_quads<true, true> : k_forward_batch_quads<true, false>)
+                      : (sky->qrec ? k_forward_batch_quads<false, true> : k_forward_batch_quads<false, false>);
+    kern<<<grid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qc2, sky->qcone, sky->nqtile, sky->rpack, sky->rcone,
+                              b->tiles, b->ntile, items, n_items, b->uvw, b->xa, b->xb, b->Ktot, b->R, b->orb, track,
+                              V);
     VISS_LAUNCHED(k_forward_batch_quads);
     return VISS_OK;
   }

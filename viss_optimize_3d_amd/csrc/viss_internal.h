@@ -176,6 +176,8 @@ struct viss_sky {
   viss::QuadPack* qpack = nullptr;  // [nqtile * kQuadTile]
   int4* qidx = nullptr;             // [nqtile * kQuadTile] caller indices of (a, b, c, d); -1 if absent
   double* qz = nullptr;             // [nqtile] z of each tile
+  double* qc2 = nullptr;            // [nqtile] 2 cos of the azimuth step of each tile's quads
+  bool qrec = false;                // every tile's phase follows the recurrence over qc2
   float4* qcone = nullptr;          // [2 nqtile] cones of the a and b pixels of each tile
   int64_t nqpatch = 0;              // back-projection patches: 4 tiles (64 quads), one per ring of a band
   int4* qpatch = nullptr;           // [nqpatch] tile of each 16-lane group; -1 if absent

@@ -379,10 +379,10 @@ __device__ __forceinline__ void ring_step(float w1, float w2, float sum, float d
 }
 
 template <int MZ, int MM>
-__device__ __forceinline__ void quad_one(const QuadPack& q, double u, double v, const VisMask& mk, const QuadOcc& zz,
-                                         float& Rz, float& Jz, float& Rm, float& Jm) {
+__device__ __forceinline__ void quad_one(const QuadPack& q, double A, const VisMask& mk, const QuadOcc& zz, float& Rz,
+                                         float& Jz, float& Rm, float& Jm) {
   float sn, cs;
-  sincos_rev(rev_fp32(fma(u, q.x, v * q.y)), sn, cs);
+  sincos_rev(rev_fp32(A), sn, cs);
   bool va, vb, vc, vd;
   quad_visible<MZ, MM>(q.x, q.y, mk, zz, va, vb, vc, vd);
   ring_step<MZ>(q.Ia, q.Ib, q.Az, q.Dz, va, vb, cs, sn, Rz, Jz);
@@ -394,16 +394,31 @@ __device__ __forceinline__ void quad_one(const QuadPack& q, double u, double v, 
 // j computes: 83.8 -> 82.5 ms on the precision workload against four quads
 // loaded and then computed at a time (two ahead 83.2 ms, four ahead 85.2 ms
 // with 104 SGPRs and one wavefront per SIMD fewer).
-template <int MZ, int MM>
-__device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ tq, double u, double v,
+// REC: the tile's a pixels sit at equal azimuth steps d on one circle (a
+// HEALPix ring), so A_j = u x_j + v y_j is a sinusoid in j and
+// A_{j+1} = c2 A_j - A_{j-1} with c2 = 2 cos d: one FP64 FMA a quad instead
+// of a multiply and an FMA, and no (x, y) loads where no pixel is tested.
+// The sky takes the recurrence only when every tile's (x, y), run through it
+// in FP64, stay within 2.5e-13 of the records (k_quad_rec): a phase error
+// under 2e-9 cycles at |b| = 3000 wavelengths, against the FP32 sin/cos's
+// 1.25e-7.  Padding quads (zero weights) past the ring's end
+// continue the recurrence: bounded for |c2| <= 2, and weighted by zero.
+template <int MZ, int MM, bool REC>
+__device__ __forceinline__ void quad_tile_forward(const QuadPack* __restrict__ tq, double c2, double u, double v,
                                                   const VisMask& mk, const QuadOcc& zz, float& Rz, float& Jz,
                                                   float& Rm, float& Jm) {
   QuadPack cur = tq[0];
+  double a0 = 0.0, a1 = 0.0;
 #pragma unroll
   for (int j = 0; j < kQuadTile; ++j) {
     QuadPack nxt = cur;
     if (j + 1 < kQuadTile) nxt = tq[j + 1];
-    quad_one<MZ, MM>(cur, u, v, mk, zz, Rz, Jz, Rm, Jm);
+    const double A = (!REC || j < 2) ? fma(u, cur.x, v * cur.y) : fma(c2, a1, -a0);
+    if (REC) {
+      a0 = a1;
+      a1 = A;
+    }
+    quad_one<MZ, MM>(cur, A, mk, zz, Rz, Jz, Rm, Jm);
     cur = nxt;
   }
 }
@@ -579,11 +594,14 @@ struct QuadSums {
   }
 };
 
-// Quad tile t in ring modes (mz, mm) (not both kRingNone), into qs.
-template <bool MASK>
-__device__ __forceinline__ void quad_tile(const QuadPack* __restrict__ qp, const double* __restrict__ qz, int64_t t,
-                                          int mz, int mm, const FwdRow& r, QuadSums& qs, double& Sr, double& Si) {
+// Quad tile t in ring modes (mz, mm) (not both kRingNone), into qs; qc2 is
+// read only with REC (see quad_tile_forward).
+template <bool MASK, bool REC>
+__device__ __forceinline__ void quad_tile(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
+                                          const double* __restrict__ qc2, int64_t t, int mz, int mm, const FwdRow& r,
+                                          QuadSums& qs, double& Sr, double& Si) {
   const double z = qz[t];
+  const double c2 = REC ? qc2[t] : 0.0;
   QuadOcc zz{0.0, 0.0};
   if (MASK) {
     zz.zza = z * r.mk.za;
@@ -593,7 +611,7 @@ __device__ __forceinline__ void quad_tile(const QuadPack* __restrict__ qp, const
   float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
 #define VISS_QT(A, B)                                                                      \
   case A * kRingModes + B:                                                                 \
-    quad_tile_forward<A, B>(tq, r.u, r.v, r.mk, zz, Rz, Jz, Rm, Jm);                      \
+    quad_tile_forward<A, B, REC>(tq, c2, r.u, r.v, r.mk, zz, Rz, Jz, Rm, Jm);             \
     break;
 #define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
                        VISS_QT(A, kRingTF) VISS_QT(A, kRingFT) VISS_QT(A, kRingTest)
@@ -618,10 +636,10 @@ __device__ __forceinline__ void quad_tile(const QuadPack* __restrict__ qp, const
 // per-lane classification (about 34 vector instructions a tile, the bounds
 // under one).  The modes, and so the results, are those of the per-lane
 // classification alone.
-template <bool MASK>
+template <bool MASK, bool REC>
 __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
-                                                   const float4* __restrict__ qcone, int64_t t0, int64_t t1,
-                                                   const FwdRow& r, double& Sr, double& Si) {
+                                                   const double* __restrict__ qc2, const float4* __restrict__ qcone,
+                                                   int64_t t0, int64_t t1, const FwdRow& r, double& Sr, double& Si) {
   const uint64_t actm = wave_ballot(r.act);
   if (MASK && actm == 0) return;  // no row: every tile would be unseen
   WaveCls wb{};
@@ -638,7 +656,7 @@ __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ 
         quad_tile_modes(qcone, t, __builtin_amdgcn_readlane(code, (int)(t - tb)), r, actm, mz, mm);
         if (mz == kRingNone && mm == kRingNone) continue;
       }
-      quad_tile<MASK>(qp, qz, t, mz, mm, r, qs, Sr, Si);
+      quad_tile<MASK, REC>(qp, qz, qc2, t, mz, mm, r, qs, Sr, Si);
     }
   }
   qs.flush(Sr, Si);

@@ -83,8 +83,9 @@ __global__ __launch_bounds__(256) void k_forward_pairs(const PairPack* __restric
 // Forward over the equatorial-mirror quad layout (see forward_quad_tiles):
 // grid.y chunks [0, nqchunk) walk quad tiles, the rest walk the pairs and
 // singles left outside the quads.
-template <bool MASK>
+template <bool MASK, bool REC>
 __global__ __launch_bounds__(256) void k_forward_quads(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
+                                                       const double* __restrict__ qc2,
                                                        const float4* __restrict__ qcone, int64_t nqtile, int64_t qpc,
                                                        int64_t nqchunk, const PairPack* __restrict__ rp,
                                                        const float4* __restrict__ rcone, int64_t nrtile, int64_t rpc,
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(256) void k_forward_quads(const QuadPack* __restric
   const int64_t c = blockIdx.y;
   if (c < nqchunk) {
     const int64_t t0 = c * qpc, t1 = min(nqtile, t0 + qpc);
-    forward_quad_tiles<MASK>(qp, qz, qcone, t0, t1, r, Sr, Si);
+    forward_quad_tiles<MASK, REC>(qp, qz, qc2, qcone, t0, t1, r, Sr, Si);
   } else {
     const int64_t t0 = (c - nqchunk) * rpc, t1 = min(nrtile, t0 + rpc);
     forward_pair_tiles<MASK>(rp, rcone, nullptr, t0, t1, r, Sr, Si);
@@ -749,14 +750,11 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
     const int64_t rpc = std::max<int64_t>(qpc / 2, cdiv(sky->nrtile, 32768));
     const int64_t nqc = cdiv(sky->nqtile, qpc), nrc = cdiv(sky->nrtile, rpc);
     dim3 qgrid((unsigned)bx, (unsigned)(nqc + nrc));
-    if (mask)
-      k_forward_quads<true><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, qpc, nqc, sky->rpack,
-                                                  sky->rcone, sky->nrtile, rpc, uvw, xyza, xyzb, K, R, orb, track,
-                                                  herm, V);
-    else
-      k_forward_quads<false><<<qgrid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qcone, sky->nqtile, qpc, nqc,
-                                                   sky->rpack, sky->rcone, sky->nrtile, rpc, uvw, xyza, xyzb, K, R,
-                                                   orb, track, herm, V);
+    // <MASK, REC>: the recurrent phase where the sky's tiles admit it
+    auto* kern = mask ? (sky->qrec ? k_forward_quads<true, true> : k_forward_quads<true, false>)
+                      : (sky->qrec ? k_forward_quads<false, true> : k_forward_quads<false, false>);
+    kern<<<qgrid, 256, 0, s>>>(sky->qpack, sky->qz, sky->qc2, sky->qcone, sky->nqtile, qpc, nqc, sky->rpack,
+                               sky->rcone, sky->nrtile, rpc, uvw, xyza, xyzb, K, R, orb, track, herm, V);
     VISS_LAUNCHED(k_forward_quads);
     return VISS_OK;
   }

@@ -544,6 +544,42 @@ __global__ void k_quad_pack(const int32_t* __restrict__ order, const int2* __res
   }
 }
 
+// Phase recurrence of each quad tile (quad_tile_forward): the a pixels of a
+// HEALPix ring sit at equal azimuth steps d on one circle, so (x_j, y_j), and
+// with them A_j = u x_j + v y_j, obey q_{j+1} = c2 q_j - q_{j-1}, c2 = 2 cos d.
+// qc2[t] = 2 (p0 . p1) / |p0|^2 from the tile's first two quads (the quads of a
+// tile are a prefix of its slots, sorted by azimuth); a tile whose (x, y), run
+// through the recurrence in FP64 as the kernels run it, leave the records by
+// more than kRecTol anywhere counts in *bad, and the sky then keeps the direct
+// phase for every tile.  A HEALPix sky stays within 9e-14 (nside 64-8192).
+constexpr double kRecTol = 2.5e-13;
+__global__ void k_quad_rec(const QuadPack* __restrict__ qp, const int4* __restrict__ qi, int64_t ntiles,
+                           double* __restrict__ qc2, int32_t* __restrict__ bad) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    const QuadPack* __restrict__ tq = qp + t * kQuadTile;
+    int n = 0;
+    for (int j = 0; j < kQuadTile; ++j) n += qi[t * kQuadTile + j].x >= 0 ? 1 : 0;
+    double c2 = 0.0;
+    bool ok = true;
+    if (n >= 3) {
+      const double x0 = tq[0].x, y0 = tq[0].y, x1 = tq[1].x, y1 = tq[1].y;
+      c2 = 2.0 * (x0 * x1 + y0 * y1) / (x0 * x0 + y0 * y0);
+      ok = fabs(c2) <= 2.0;  // false for NaN too
+      double px = x0, py = y0, cx = x1, cy = y1;
+      for (int j = 2; j < n; ++j) {
+        const double nx = fma(c2, cx, -px), ny = fma(c2, cy, -py);
+        ok = ok & (fabs(nx - tq[j].x) <= kRecTol) & (fabs(ny - tq[j].y) <= kRecTol);
+        px = cx;
+        py = cy;
+        cx = nx;
+        cy = ny;
+      }
+    }
+    qc2[t] = ok ? c2 : 0.0;
+    if (!ok) atomicAdd(bad, 1);
+  }
+}
+
 // Cones of the a pixels (x, y, z) and b pixels (-x, -y, z) of each tile; the
 // d and c pixels are their antipodes (the negated cones).
 __global__ void k_quad_cones(const QuadPack* __restrict__ qp, const int4* __restrict__ qi,
@@ -760,6 +796,7 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       VISS_HIP(hipMalloc((void**)&sky->qpack, sizeof(QuadPack) * nslot));
       VISS_HIP(hipMalloc((void**)&sky->qidx, sizeof(int4) * nslot));
       VISS_HIP(hipMalloc((void**)&sky->qz, sizeof(double) * nqt));
+      VISS_HIP(hipMalloc((void**)&sky->qc2, sizeof(double) * nqt));
       VISS_HIP(hipMalloc((void**)&sky->qcone, sizeof(float4) * 2 * nqt));
       k_quad_clear<<<grid_for(nslot, 256), 256, 0, s>>>(sky->qpack, sky->qidx, nslot);
       VISS_LAUNCHED(k_quad_clear);
@@ -770,6 +807,9 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       VISS_LAUNCHED(k_quad_pack);
       k_quad_cones<<<grid_for(nqt, 256), 256, 0, s>>>(sky->qpack, sky->qidx, sky->qz, nqt, sky->qcone);
       VISS_LAUNCHED(k_quad_cones);
+      VISS_HIP(hipMemsetAsync(counts + 3, 0, sizeof(int32_t), s));
+      k_quad_rec<<<grid_for(nqt, 256), 256, 0, s>>>(sky->qpack, sky->qidx, nqt, sky->qc2, counts + 3);
+      VISS_LAUNCHED(k_quad_rec);
       // Back-projection patches: bands of 4 consecutive rings (in z order);
       // patch j of a band takes azimuth tile j of each ring, so a wavefront's
       // 64 quads stay compact.
@@ -808,10 +848,11 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       VISS_LAUNCHED(k_rest_flags);
       b = tmp_bytes;
       VISS_HIP(hipcub::DeviceSelect::Flagged(tmp, b, iota, rest, rest_pairs, counts + 2, (int)np, s));
-      int32_t hr = 0;
-      VISS_HIP(hipMemcpyAsync(&hr, counts + 2, sizeof(hr), hipMemcpyDeviceToHost, s));
+      int32_t hr[2] = {0, 0};  // rest pairs, tiles off the phase recurrence
+      VISS_HIP(hipMemcpyAsync(hr, counts + 2, sizeof(hr), hipMemcpyDeviceToHost, s));
       VISS_HIP(hipStreamSynchronize(s));
-      const int64_t nrp = hr, nrp_pad = (nrp + 63) / 64 * 64, ns_pad = (sky->nsingle + 63) / 64 * 64;
+      sky->qrec = hr[1] == 0;
+      const int64_t nrp = hr[0], nrp_pad = (nrp + 63) / 64 * 64, ns_pad = (sky->nsingle + 63) / 64 * 64;
       const int64_t single0 = (sky->npair + 63) / 64 * 64;
       sky->nrtile = (nrp_pad + ns_pad) / 64;
       if (nrp != np - 2 * nq) {
@@ -893,14 +934,16 @@ void sky_free(viss_sky* sky) {
   sky->ppack = nullptr;
   sky->pidx = nullptr;
   sky->pcone = nullptr;
-  for (void* p : {(void*)sky->qpack, (void*)sky->qidx, (void*)sky->qz, (void*)sky->qcone, (void*)sky->qpatch,
-                  (void*)sky->qpcone, (void*)sky->rpack, (void*)sky->ridx, (void*)sky->rcone})
+  for (void* p : {(void*)sky->qpack, (void*)sky->qidx, (void*)sky->qz, (void*)sky->qc2, (void*)sky->qcone,
+                  (void*)sky->qpatch, (void*)sky->qpcone, (void*)sky->rpack, (void*)sky->ridx, (void*)sky->rcone})
     if (p) (void)hipFree(p);
   sky->qpatch = nullptr;
   sky->qpcone = nullptr;
   sky->qpack = nullptr;
   sky->qidx = nullptr;
   sky->qz = nullptr;
+  sky->qc2 = nullptr;
+  sky->qrec = false;
   sky->qcone = nullptr;
   sky->rpack = nullptr;
   sky->ridx = nullptr;
@@ -992,7 +1035,7 @@ int viss_sky_quads(const viss_sky* sky, int64_t* nquad, int64_t* nquad_tiles, in
   VISS_REQUIRE(sky != nullptr, "viss_sky_quads: sky is NULL");
   if (nquad) *nquad = sky->nquad;
   if (nquad_tiles) *nquad_tiles = sky->nqtile;
-  if (uses_quads) *uses_quads = sky->use_quads ? 1 : 0;
+  if (uses_quads) *uses_quads = !sky->use_quads ? 0 : sky->qrec ? 2 : 1;
   return VISS_OK;
 }
 

@@ -101,7 +101,8 @@ class Sky:
     def pairs(self):
         """Antipodal pairs and unpaired pixels found at creation, and whether the
         mixed per-row kernels use the pairs; equatorial-mirror quads, their
-        16-quad tiles, and whether the mixed per-row forward uses them."""
+        16-quad tiles, whether the mixed per-row forward uses them, and whether
+        it forms their phases by the azimuth recurrence (HEALPix rings)."""
         n_pair, n_single, uses = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
         check("viss_sky_pairs", _lib.load().viss_sky_pairs(self._h, ctypes.byref(n_pair), ctypes.byref(n_single),
                                                            ctypes.byref(uses)))
@@ -109,7 +110,8 @@ class Sky:
         n_quad, n_qtile, uses_q = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
         check("viss_sky_quads", _lib.load().viss_sky_quads(self._h, ctypes.byref(n_quad), ctypes.byref(n_qtile),
                                                            ctypes.byref(uses_q)))
-        out.update({"quads": n_quad.value, "quad_tiles": n_qtile.value, "uses_quads": bool(uses_q.value)})
+        out.update({"quads": n_quad.value, "quad_tiles": n_qtile.value, "uses_quads": bool(uses_q.value),
+                    "quad_recurrence": uses_q.value == 2})
         return out
 
     def close(self):
