@@ -82,9 +82,11 @@ __global__ __launch_bounds__(256) void k_forward_pairs(const PairPack* __restric
 
 // Forward over the equatorial-mirror quad layout (see forward_quad_tiles):
 // grid.y chunks [0, nqchunk) walk quad tiles, the rest walk the pairs and
-// singles left outside the quads.
+// singles left outside the quads.  Held to 8 wavefronts per SIMD (64 VGPRs):
+// the masked recurrent instance would take 66 and run 7; it spills 20 B a
+// lane instead and runs the precision workload in 74.1 against 75.4 ms.
 template <bool MASK, bool REC>
-__global__ __launch_bounds__(256) void k_forward_quads(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_forward_quads(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
                                                        const double* __restrict__ qc2,
                                                        const float4* __restrict__ qcone, int64_t nqtile, int64_t qpc,
                                                        int64_t nqchunk, const PairPack* __restrict__ rp,
