@@ -57,23 +57,27 @@ PAIR_FP64 = 6
 PAIR_FP32 = 34
 PAIR_T_PEAK = PAIR_FP32 / PEAK_FP32 + PAIR_FP64 / PEAK_FP64  # s per pair at peak
 # Algorithmic work of the quad kernels (one sin/cos per quad of pixels
-# (+-x, +-y, +-z), the ring factor once per 16-quad tile), per unit counted by
-# viss_sky_count_work with the kernels' own occultation test; sin + cos count
-# 20 FP32 flops as in the per-pair convention, an FMA 2:
+# (+-x, +-y, +-z), the ring factor once per run of same-ring tiles), per unit
+# counted by viss_sky_count_work with the kernels' own occultation test; sin +
+# cos count 20 FP32 flops as in the per-pair convention, an FMA 2:
 #   (row, quad) with a visible pixel:   phase u x + v y (mul + FMA: 3 FP64) and its
 #                                       mod-1 reduction (1 FP64), sin + cos (20 FP32)
 #   (row, quad, ring) with a visible pixel: the ring's two multiply-adds (4 FP32)
-#   (row, tile) with a visible pixel:   ring phase w z and its reduction (2 FP64), sin + cos
-#                                       (20 FP32), two complex products into the row sum (12 FP32)
+#   (row, tile) with a visible pixel:   its four ring sums join the run's (4 FP32)
+#   (row, run of up to 8 such tiles on one ring pair): ring phase w z and its
+#                                       reduction (2 FP64), sin + cos (20 FP32), two
+#                                       complex products into the row sum (12 FP32)
 #   (row, rest pair) with a visible pixel: phase (5 FP64) + reduction (1), sin + cos (20),
 #                                       two multiply-adds (4 FP32)
 # The occultation classification and per-pixel tests are not counted: they decide
 # which of these units run, and their cost shows in the VALU issue fraction.
-QUAD_WORK = {"row_quads": (4, 20), "row_rings": (0, 4), "row_tiles": (2, 32), "row_rest_pairs": (6, 24)}
+QUAD_WORK = {"row_quads": (4, 20), "row_rings": (0, 4), "row_tiles": (0, 4), "row_runs": (2, 32),
+             "row_rest_pairs": (6, 24)}
 # Skies whose tiles take the recurrent phase (HEALPix rings: equal azimuth steps):
 # A_{j+1} = 2 cos(d) A_j - A_{j-1}, one FMA (2 FP64) per (row, quad) from a tile's
 # third quad on; the first two keep mul + FMA, charged per (row, tile) (+2 FP64).
-QUAD_WORK_REC = {"row_quads": (3, 20), "row_rings": (0, 4), "row_tiles": (4, 32), "row_rest_pairs": (6, 24)}
+QUAD_WORK_REC = {"row_quads": (3, 20), "row_rings": (0, 4), "row_tiles": (2, 4), "row_runs": (2, 32),
+                 "row_rest_pairs": (6, 24)}
 
 
 def quad_flops(work, rec=False):

@@ -212,12 +212,14 @@ int viss_sky_count_visible(viss_sky* sky, const double* xyz_a, const double* xyz
 
 /* Work of the mixed-precision per-row kernels over a sky's quad layout for K
  * masked rows, counted with the kernels' own occultation test; accumulated
- * into counts (device int64 [5]):
+ * into counts (device int64 [6]):
  *   [0] visible (row, pixel) pairs (= viss_sky_count_visible),
- *   [1] (row, 16-quad tile) with a visible pixel: one ring factor each,
+ *   [1] (row, 16-quad tile) with a visible pixel,
  *   [2] (row, quad) with a visible pixel: one phase and one sin/cos each,
  *   [3] (row, quad, ring z or -z) with a visible pixel: one pair of FMAs each,
- *   [4] (row, pair outside the quads) with a visible pixel: one phase and sin/cos.
+ *   [4] (row, pair outside the quads) with a visible pixel: one phase and sin/cos,
+ *   [5] (row, run of up to 8 consecutive such tiles on one ring pair, within
+ *       one of the forward's tile chunks): one ring factor each.
  * The measurement of the kernels' algorithmic work (no counterpart in the
  * reference).  VISS_ERR_INVALID for a sky without the quad layout. */
 int viss_sky_count_work(viss_sky* sky, const double* xyz_a, const double* xyz_b, int64_t K,

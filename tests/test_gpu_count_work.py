@@ -8,7 +8,11 @@ lies within rounding of a cut, so the evaluation order does not matter), the
 visible pixels, the (row, quad) and (row, quad, ring) units with a visible
 pixel and the (row, equator pair) units must equal the device's counts
 exactly; a 16-quad tile with a visible pixel is counted at most once per row,
-so the tile count lies between the quad count / 16 and the quad count.
+so the tile count lies between the quad count / 16 and the quad count.  The
+tiles and the runs that share a ring factor are rebuilt on the host as well
+(quads sorted by ring height, then azimuth, 16 to a tile; runs of up to 8
+consecutive tiles with a visible pixel on one ring within a chunk of the
+forward's), so both counts are checked exactly too.
 """
 import numpy as np
 import pytest
@@ -20,6 +24,54 @@ from tests.cases import make_case
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 R = 1737.4e3
+
+
+def host_tiles(lmn, quads):
+    """Tile index (z, then azimuth order, 16 quads a tile) of every quad, and the
+    ring height of every tile."""
+    x, y, z = lmn[:, quads[:, 0]]
+    bx, by = lmn[0, quads[:, 1]], lmn[1, quads[:, 1]]
+    up = np.where(np.abs(y) > 1e-9, y > 0, x > 0)  # the quad's pixel in the upper half-plane
+    ux, uy = np.where(up, x, bx), np.where(up, y, by)
+    phi = np.maximum(np.arctan2(uy, ux) + 0.0, 0.0)
+    order = np.lexsort((phi, z))
+    tile_of = np.empty(len(quads), np.int64)
+    tile_z = []
+    t = 0
+    i = 0
+    while i < len(order):
+        j = i
+        while j < len(order) and z[order[j]] == z[order[i]]:
+            j += 1
+        for s in range(i, j, 16):
+            tile_of[order[s:min(j, s + 16)]] = t
+            tile_z.append(z[order[i]])
+            t += 1
+        i = j
+    return tile_of, np.array(tile_z)
+
+
+def host_runs(vis_tiles, tile_z, qpc):
+    """Runs of the forward for rows on their own: per chunk of qpc tiles, up to 8
+    consecutive tiles with a visible pixel on one ring."""
+    runs = 0
+    for row in vis_tiles:
+        for c0 in range(0, len(tile_z), qpc):
+            zrun, n = None, 0
+            for t in np.flatnonzero(row[c0:c0 + qpc]) + c0:
+                if tile_z[t] != zrun or n == 8:
+                    runs += 1
+                    zrun, n = tile_z[t], 0
+                n += 1
+    return runs
+
+
+def quad_chunk(K, nqtile, nrtile):
+    """Quad tiles per chunk of the per-row forward (its launch rule)."""
+    bx, qpc = -(-K // 256), 512
+    while qpc > 32 and bx * (-(-nqtile // qpc) + -(-nrtile // (qpc // 2))) < 4096:
+        qpc //= 2
+    return max(qpc, -(-nqtile // 32768))
 
 
 def host_quads(lmn):
@@ -58,8 +110,15 @@ def test_count_work_matches_host(nside, nest):
     want = {"visible_pairs": int(vis.sum()), "row_quads": int(q.any(axis=2).sum()),
             "row_rings": int((q[:, :, 0] | q[:, :, 1]).sum() + (q[:, :, 2] | q[:, :, 3]).sum()),
             "row_rest_pairs": int(vis[:, pairs].any(axis=2).sum())}
+    tile_of, tile_z = host_tiles(lmn, quads)
+    vis_tiles = np.zeros((K, len(tile_z)), bool)
+    for k in range(K):
+        vis_tiles[k, tile_of[q[k].any(axis=1)]] = True
+    want["row_tiles"] = int(vis_tiles.sum())
+    nrtile = -(-pairs.shape[0] // 64)  # the equator pairs; a HEALPix map has no singles
+    want["row_runs"] = host_runs(vis_tiles, tile_z, quad_chunk(K, len(tile_z), nrtile))
     sky = ops.Sky(torch.from_numpy(lmn).cuda())
-    assert sky.pairs()["quads"] == quads.shape[0]
+    assert sky.pairs()["quads"] == quads.shape[0] and sky.pairs()["quad_tiles"] == len(tile_z)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
     got = sky.count_work(t(c["xa"]), t(c["xb"]), R_moon=R)
     print(f"nside {nside} {'NEST' if nest else 'RING'}: {got}", flush=True)
@@ -67,6 +126,7 @@ def test_count_work_matches_host(nside, nest):
         assert got[key] == v, (key, got[key], v)
     assert got["visible_pairs"] == int(sky.count_visible(t(c["xa"]), t(c["xb"]), R_moon=R).item())
     assert want["row_quads"] / 16 <= got["row_tiles"] <= want["row_quads"]
+    assert got["row_tiles"] / 8 <= got["row_runs"] <= got["row_tiles"]
 
 
 def test_count_work_needs_quads():

@@ -621,11 +621,14 @@ __global__ __launch_bounds__(256) void k_count_visible(const PixPack* __restrict
 // ------------------------------------------------------------ work count of the quad layout
 // The work of the mixed per-row kernels over the quad layout, counted with
 // their own occultation test (quad_visible, occ_dot): per row, the visible
-// pixels, the quad tiles with a visible pixel (one ring factor each), the
-// quads with a visible pixel (one phase and one sin/cos each), the rings of a
-// quad with a visible pixel (one pair of multiply-adds each), and the pairs of
-// the rest layout with a visible pixel (one phase and sin/cos each).  One row
-// per lane; grid.y chunks of quad tiles, then of rest tiles.
+// pixels, the quad tiles with a visible pixel, the runs of such tiles that
+// share one ring factor (RingRun: up to kRunTiles consecutive tiles of one
+// ring pair within a chunk of the forward's), the quads with a visible pixel
+// (one phase and one sin/cos each), the rings of a quad with a visible pixel
+// (one pair of multiply-adds each), and the pairs of the rest layout with a
+// visible pixel (one phase and sin/cos each).  One row per lane; grid.y
+// chunks of quad tiles (those of the forward, so the runs are its runs for a
+// row on its own), then of rest tiles.
 __global__ __launch_bounds__(256) void k_count_quad_work(
     const QuadPack* __restrict__ qp, const int4* __restrict__ qi, const double* __restrict__ qz, int64_t nqtile,
     int64_t qpc, int64_t nqchunk, const PairPack* __restrict__ rp, const int2* __restrict__ ri, int64_t nrtile,
@@ -638,10 +641,12 @@ __global__ __launch_bounds__(256) void k_count_quad_work(
   VisCls cl;
   make_mask_records(xyza[kk], xyza[K + kk], xyza[2 * K + kk], xyzb[kk], xyzb[K + kk], xyzb[2 * K + kk], R_moon,
                     orbit_r, mk, cl);
-  unsigned long long npix = 0, ntile = 0, nquad = 0, nring = 0, npair = 0;
+  unsigned long long npix = 0, ntile = 0, nquad = 0, nring = 0, npair = 0, nrun = 0;
   const int64_t c = blockIdx.y;
   if (c < nqchunk) {
     const int64_t t1 = min(nqtile, (c + 1) * qpc);
+    uint64_t zrun = ~0ull;
+    int in_run = 0;
     for (int64_t t = c * qpc; t < t1; ++t) {
       const double z = qz[t];
       const QuadOcc zz{z * mk.za, z * mk.zb};
@@ -662,6 +667,15 @@ __global__ __launch_bounds__(256) void k_count_quad_work(
         tq += any;
       }
       ntile += tq > 0;
+      if (tq > 0) {  // as forward_quad_tiles forms its runs
+        const uint64_t zb = (uint64_t)__double_as_longlong(z);
+        if (zb != zrun || in_run == kRunTiles) {
+          ++nrun;
+          zrun = zb;
+          in_run = 0;
+        }
+        ++in_run;
+      }
     }
   } else {
     const int64_t r0 = (c - nqchunk) * rpc, r1 = min(nrtile, r0 + rpc);
@@ -676,10 +690,10 @@ __global__ __launch_bounds__(256) void k_count_quad_work(
       npair += va | vb;
     }
   }
-  if (!act) npix = ntile = nquad = nring = npair = 0;
-  unsigned long long v[5] = {npix, ntile, nquad, nring, npair};
+  if (!act) npix = ntile = nquad = nring = npair = nrun = 0;
+  unsigned long long v[6] = {npix, ntile, nquad, nring, npair, nrun};
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < 6; ++i) {
     for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o, 64);
     if ((threadIdx.x & 63) == 0 && v[i]) atomicAdd(&counts[i], v[i]);
   }
@@ -697,6 +711,20 @@ static int64_t env_int(const char* name, int64_t dflt) {
   if (!v || !*v) return dflt;
   const long long x = atoll(v);
   return x > 0 ? (int64_t)x : dflt;
+}
+
+// Chunks of the per-row quad forward over bx blocks of rows: qpc quad tiles
+// (64 pixels each) per chunk, 512 by default, which keeps the 32768 pixels of
+// 256 pair tiles; rpc rest tiles (64 pairs) per chunk.  Halved while the grid
+// would hold under kMinBlocks blocks; at most 32768 chunks of each kind.
+static void quad_fwd_chunks(const viss_sky* sky, int64_t bx, int64_t& qpc, int64_t& rpc) {
+  qpc = env_int("VISS_FWD_QTILES", 0);
+  if (qpc <= 0) {
+    qpc = 2 * kFwdTilesPerChunk;
+    while (qpc > 32 && bx * (cdiv(sky->nqtile, qpc) + cdiv(sky->nrtile, qpc / 2)) < kMinBlocks) qpc /= 2;
+  }
+  qpc = std::max(qpc, cdiv(sky->nqtile, 32768));
+  rpc = std::max<int64_t>(qpc / 2, cdiv(sky->nrtile, 32768));
 }
 
 static int validate_geom(const viss_mask_geometry* g, int flags) {
@@ -741,15 +769,8 @@ static int sky_forward_impl(viss_sky* sky, const double* I, const double* uvw, c
   dim3 grid((unsigned)bx, (unsigned)cdiv(ntile, tpc));
   const int track = (flags & VISS_PHASE_TRACK) ? 1 : 0, herm = (flags & VISS_HERMITIAN) ? 1 : 0;
   if (!f64 && sky->use_quads) {
-    // a quad tile holds 64 pixels: 512 per chunk keeps the 32768 pixels of
-    // 256 pair tiles; rest tiles (64 pairs) go 256 per chunk
-    int64_t qpc = env_int("VISS_FWD_QTILES", 0);
-    if (qpc <= 0) {
-      qpc = 2 * kFwdTilesPerChunk;
-      while (qpc > 32 && bx * (cdiv(sky->nqtile, qpc) + cdiv(sky->nrtile, qpc / 2)) < kMinBlocks) qpc /= 2;
-    }
-    qpc = std::max(qpc, cdiv(sky->nqtile, 32768));
-    const int64_t rpc = std::max<int64_t>(qpc / 2, cdiv(sky->nrtile, 32768));
+    int64_t qpc = 0, rpc = 0;
+    quad_fwd_chunks(sky, bx, qpc, rpc);
     const int64_t nqc = cdiv(sky->nqtile, qpc), nrc = cdiv(sky->nrtile, rpc);
     dim3 qgrid((unsigned)bx, (unsigned)(nqc + nrc));
     // <MASK, REC>: the recurrent phase where the sky's tiles admit it
@@ -1027,7 +1048,8 @@ int viss_sky_count_work(viss_sky* sky, const double* xyz_a, const double* xyz_b,
   if (rc != VISS_OK) return rc;
   const int64_t bx = cdiv(K, 256);
   VISS_REQUIRE(bx <= INT32_MAX, "viss_sky_count_work: K too large for one launch");
-  const int64_t qpc = std::max<int64_t>(64, cdiv(sky->nqtile, 32768)), rpc = std::max<int64_t>(8, cdiv(sky->nrtile, 32768));
+  int64_t qpc = 0, rpc = 0;
+  quad_fwd_chunks(sky, bx, qpc, rpc);
   const int64_t nqc = cdiv(sky->nqtile, qpc), nrc = cdiv(sky->nrtile, rpc);
   dim3 grid((unsigned)bx, (unsigned)(nqc + nrc));
   k_count_quad_work<<<grid, 256, 0, as_stream(stream)>>>(sky->qpack, sky->qidx, sky->qz, sky->nqtile, qpc, nqc,

@@ -238,11 +238,12 @@ class Sky:
         L = _lib.load()
         xa = _dev(xyz_a, torch.float64)
         xb = _dev(xyz_b, torch.float64)
-        cnt = torch.zeros(5, dtype=torch.int64, device=self.device)
+        cnt = torch.zeros(6, dtype=torch.int64, device=self.device)
         check("viss_sky_count_work", L.viss_sky_count_work(self._h, _p(xa), _p(xb), xa.shape[1],
                                                            _geom(R_moon, orbit_radius), _p(cnt), _stream()))
         c = cnt.cpu().tolist()
-        return {"visible_pairs": c[0], "row_tiles": c[1], "row_quads": c[2], "row_rings": c[3], "row_rest_pairs": c[4]}
+        return {"visible_pairs": c[0], "row_tiles": c[1], "row_quads": c[2], "row_rings": c[3], "row_rest_pairs": c[4],
+                "row_runs": c[5]}
 
 
 def weights(bll: torch.Tensor, w: torch.Tensor, nr: int, blockage_shift: bool = False, want_dg: bool = True):
