@@ -239,6 +239,12 @@ __global__ __launch_bounds__(256) void k_forward_batch_pairs(const PairPack* __r
 // gives the kernel over 100 VGPRs (4 wavefronts per SIMD).  Held to 7
 // wavefronts it fits in 66 without spilling (8 would spill to scratch): at
 // 4.48e6 rows x 196,608 pixels, 102.9 ms per pass against 128 ms at 4.
+// Quad tiles join ring-pair runs as in forward_quad_tiles, but a run is cut
+// by list position: it holds the evaluated tiles of one ring pair among
+// positions [8 m, 8 m + 8).  Items start at multiples of 8 (build_items), so
+// no run crosses an item and the FP32 sums are the same for every work
+// table: results differ between tables (autotuned or not) by the order of
+// the FP64 sums only.
 template <bool MASK, bool REC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_forward_batch_quads(const QuadPack* __restrict__ qp,
                                                              const double* __restrict__ qz,
@@ -265,11 +271,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   WaveCls wb{};
   if (MASK) wb = wave_cls(r.cl, r.act);
   double Sr = 0.0, Si = 0.0;
-  QuadSums qs;
+  RingRun rr;
+  int64_t run_blk = -1;  // list position / 8 of the open run
   // list entries 64 at a time: each lane classifies one quad tile against the
   // item's bounds (as forward_quad_tiles), then the wavefront walks them.
-  // Each tile's FP32 sum joins the FP64 sums at once, so results differ
-  // between work tables (autotuned or not) by FP64 rounding only.
   for (int64_t jb = w.j0; jb < w.j1; jb += 64) {
     const int64_t je = min(w.j1, jb + 64);
     int code = kTileExact;
@@ -285,14 +290,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
           quad_tile_modes(qcone, t, __builtin_amdgcn_readlane(code, (int)(j - jb)), r, actm, mz, mm);
           if (mz == kRingNone && mm == kRingNone) continue;
         }
-        quad_tile<MASK, REC>(qp, qz, qc2, t, mz, mm, r, qs, Sr, Si);
-        qs.flush(Sr, Si);  // per tile: the sums do not depend on how the table cuts the list
+        const double z = qz[t];
+        if ((uint64_t)__double_as_longlong(z) != rr.zbits || (j >> 3) != run_blk) {  // wave-uniform
+          if (rr.n > 0) rr.close(r, Sr, Si);
+          rr.open(z);
+          run_blk = j >> 3;
+        }
+        quad_tile_run<REC>(qp, qc2, t, mz, mm, r, rr);
+        ++rr.n;
       } else {
         forward_pair_tiles<MASK>(rp, rcone, nullptr, t - nqtile, t - nqtile + 1, r, Sr, Si);
       }
     }
   }
-  qs.flush(Sr, Si);
+  if (rr.n > 0) rr.close(r, Sr, Si);
   if (r.act) forward_pair_store(V, k, Ktot, 0, r.c0, Sr, Si);
 }
 
@@ -368,6 +379,7 @@ static void build_items(const viss_batch* b, const std::vector<int>& groups, std
       auto it = tune.find(bucket_of(nt, rows));
       tpc = it != tune.end() ? it->second : default_tpc(nt);
     }
+    tpc = (int)std::min<int64_t>(((int64_t)tpc + 7) / 8 * 8, 1 << 30);  // items start at multiples of 8
     for (int64_t r0 = 0; r0 < rows; r0 += 64)
       for (int64_t j = 0; j < nt; j += tpc) {
         WorkItem w;

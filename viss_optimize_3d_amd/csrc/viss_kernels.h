@@ -578,61 +578,6 @@ __device__ __forceinline__ void quad_tile_modes(const float4* __restrict__ qcone
   }
 }
 
-// Running sums of the quad tiles of one row: the ring-factor products of up
-// to 8 tiles are summed in FP32 before they join the FP64 sums.
-struct QuadSums {
-  float fr = 0.f, fi = 0.f;
-  int nacc = 0;
-  // adds the tile's e^{-i C}(Rz - i Jz) + e^{+i C}(Rm - i Jm), C's sin / cos = (sc, cc)
-  __device__ __forceinline__ void add(float sc, float cc, float Rz, float Jz, float Rm, float Jm, double& Sr,
-                                      double& Si) {
-    fr = fmaf(cc, Rz + Rm, fmaf(sc, Jm - Jz, fr));
-    fi = fmaf(sc, Rz - Rm, fmaf(cc, Jz + Jm, fi));
-    if (++nacc == 8) {
-      Sr += (double)fr;
-      Si += (double)fi;
-      fr = fi = 0.f;
-      nacc = 0;
-    }
-  }
-  __device__ __forceinline__ void flush(double& Sr, double& Si) {
-    Sr += (double)fr;
-    Si += (double)fi;
-    fr = fi = 0.f;
-    nacc = 0;
-  }
-};
-
-// Quad tile t in ring modes (mz, mm) (not both kRingNone), into qs; qc2 is
-// read only with REC (see quad_tile_forward).
-template <bool MASK, bool REC>
-__device__ __forceinline__ void quad_tile(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
-                                          const double* __restrict__ qc2, int64_t t, int mz, int mm, const FwdRow& r,
-                                          QuadSums& qs, double& Sr, double& Si) {
-  const double z = qz[t];
-  const double c2 = REC ? qc2[t] : 0.0;
-  const QuadPack* __restrict__ tq = qp + t * kQuadTile;
-  float Rz = 0.f, Jz = 0.f, Rm = 0.f, Jm = 0.f;
-#define VISS_QT(A, B)                                                                      \
-  case A * kRingModes + B:                                                                 \
-    quad_tile_forward<A, B, REC>(tq, c2, r.u, r.v, r.mk, z, Rz, Jz, Rm, Jm);              \
-    break;
-#define VISS_QT_ROW(A) VISS_QT(A, kRingNone) VISS_QT(A, kRingFF) VISS_QT(A, kRingFN) VISS_QT(A, kRingNF) \
-                       VISS_QT(A, kRingTF) VISS_QT(A, kRingFT) VISS_QT(A, kRingTest)
-  switch (mz * kRingModes + mm) {  // wave-uniform
-    VISS_QT(kRingNone, kRingFF) VISS_QT(kRingNone, kRingFN) VISS_QT(kRingNone, kRingNF)
-    VISS_QT(kRingNone, kRingTF) VISS_QT(kRingNone, kRingFT) VISS_QT(kRingNone, kRingTest)
-    VISS_QT_ROW(kRingFF) VISS_QT_ROW(kRingFN) VISS_QT_ROW(kRingNF) VISS_QT_ROW(kRingTF) VISS_QT_ROW(kRingFT)
-    VISS_QT_ROW(kRingTest)
-    default: break;
-  }
-#undef VISS_QT_ROW
-#undef VISS_QT
-  float sc, cc;
-  sincos_rev(rev_fp32(r.w * z), sc, cc);
-  qs.add(sc, cc, Rz, Jz, Rm, Jm, Sr, Si);
-}
-
 // A run of up to kRunTiles evaluated quad tiles on one ring pair (the same
 // z): each tile's ring sums (16 quads, FP32) join the run's sums Rz, Jz, Rm,
 // Jm (FP32), and the ring factor (one sin/cos of w z and two complex
