@@ -112,6 +112,17 @@ constexpr int kPadTo = kBpTile;  // sky padding granularity
 // per-lane vector loads that would each hold a copy in VGPRs.
 __device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// The lane's index in its wavefront, formed afresh (two VALU instructions)
+// where it is used.  The asm is volatile so that the compiler neither hoists
+// it nor merges it with another copy: a lane index computed once and kept
+// live across a kernel is one more VGPR, and in a kernel held to 64 VGPRs it
+// is the one spilled to scratch.
+__device__ __forceinline__ int lane_fresh() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
+
 // Occultation threshold t(r): s visible from r iff s.r >= t
 // (vissgen3d_noblockage.cu:88-94,240-243 in dot-product form).  Written with
 // explicit FMAs so that no compiler contraction changes its rounding: the

@@ -653,11 +653,11 @@ __device__ __forceinline__ void forward_quad_tiles(const QuadPack* __restrict__ 
   if (MASK && actm == 0) return;  // no row: every tile would be unseen
   WaveCls wb{};
   if (MASK) wb = wave_cls(r.cl, r.act);
-  const int lane = threadIdx.x & 63;
   RingRun rr;
   for (int64_t tb = t0; tb < t1; tb += 64) {
     const int64_t te = min(t1, tb + 64);
     int code = kTileExact;
+    const int lane = lane_fresh();
     if (MASK && tb + lane < te) code = quad_tile_code(qcone[2 * (tb + lane)], qcone[2 * (tb + lane) + 1], wb);
     for (int64_t t = tb; t < te; ++t) {
       int mz = kRingFF, mm = kRingFF;

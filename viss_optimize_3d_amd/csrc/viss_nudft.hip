@@ -83,8 +83,11 @@ __global__ __launch_bounds__(256) void k_forward_pairs(const PairPack* __restric
 // Forward over the equatorial-mirror quad layout (see forward_quad_tiles):
 // grid.y chunks [0, nqchunk) walk quad tiles, the rest walk the pairs and
 // singles left outside the quads.  Held to 8 wavefronts per SIMD (64 VGPRs):
-// the masked recurrent instance would take 66 and run 7; it spills 20 B a
-// lane instead and runs the precision workload in 74.1 against 75.4 ms.
+// left alone the masked recurrent instance takes 66 and runs 7 (75.4 ms on
+// the precision workload against 74.1 at 8).  Squeezed into 64 it spilled the
+// row index and the lane index to scratch (20 B a lane, 0.35 GB of writes a
+// launch); both are formed afresh where they are used (lane_fresh), and it
+// fits in 63 without spilling.
 template <bool MASK, bool REC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_forward_quads(const QuadPack* __restrict__ qp, const double* __restrict__ qz,
                                                        const double* __restrict__ qc2,
@@ -94,7 +97,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                                                        const double* __restrict__ uvw, const double* __restrict__ xyza,
                                                        const double* __restrict__ xyzb, int64_t K, double R_moon,
                                                        double orbit_r, int track, int herm, double* __restrict__ V) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t kbase = (int64_t)blockIdx.x * 256 + 64 * wave_in_block();  // wave-uniform
+  const int64_t k = kbase + lane_fresh();
   const FwdRow r = load_fwd_row<MASK>(uvw, xyza, xyzb, K, k, k < K, R_moon, orbit_r, track);
   double Sr = 0.0, Si = 0.0;
   const int64_t c = blockIdx.y;
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     const int64_t t0 = (c - nqchunk) * rpc, t1 = min(nrtile, t0 + rpc);
     forward_pair_tiles<MASK>(rp, rcone, nullptr, t0, t1, r, Sr, Si);
   }
-  if (r.act) forward_pair_store(V, k, K, herm, r.c0, Sr, Si);
+  if (r.act) forward_pair_store(V, kbase + lane_fresh(), K, herm, r.c0, Sr, Si);  // the row index, formed again
 }
 
 // ------------------------------------------------------------ back-projection
