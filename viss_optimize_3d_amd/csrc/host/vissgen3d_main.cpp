@@ -286,8 +286,9 @@ int main(int argc, char** argv) {
     int use_pairs = 0, use_quads = 0;
     VISSCHK(viss_sky_pairs(sky, &npair, &nsingle, &use_pairs));
     VISSCHK(viss_sky_quads(sky, &nquad, &nqt, &use_quads));
-    printf("sky: %lld pixels, %lld antipodal pairs, %lld mirror quads: %s kernels\n", (long long)P,
-           (long long)npair, (long long)nquad, use_quads ? "quad" : (use_pairs ? "pair" : "per-pixel"));
+    printf("sky: %lld pixels, %lld antipodal pairs, %lld mirror quads: %s kernels%s\n", (long long)P,
+           (long long)npair, (long long)nquad, use_quads ? "quad" : (use_pairs ? "pair" : "per-pixel"),
+           use_quads == 2 ? " (recurrent phase)" : "");
   }
 
   double *d_uvw = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_uvwf = nullptr, *d_xaf = nullptr, *d_xbf = nullptr;
