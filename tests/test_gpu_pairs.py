@@ -218,3 +218,32 @@ def test_quad_layouts_against_oracle(ops, kind):
         rx = rel_l2(x.cpu().numpy(), xo)
         print(f"{kind} mask={mask} track={track} herm={herm}: forward {rv:.2e}, back-projection {rx:.2e}", flush=True)
         assert rv <= 1e-5 and rx <= 1e-5
+
+
+def test_quad_recurrence_matches_direct_phase(ops, monkeypatch):
+    """The quad forward's recurrent phase (A_{j+1} = 2 cos(d) A_j - A_{j-1} along a
+    tile's quads) against its direct phase (u x + v y per quad) on one HEALPix
+    sky, nside 64 RING, with masked, phase-tracked and unmasked rows.  The phases
+    differ by under 2e-9 cycles, so the two V differ only where that moves an FP32
+    argument by an ulp: within 1e-7 relative L2 (the mixed path's own error
+    against FP64 is about 2e-7), and each within 1e-6 of the all-FP64 kernels."""
+    th, ph, _, _ = oracle.pix2ang(64, False)
+    lmn = torch.from_numpy(np.stack(oracle.pixel_dirs(th, ph, np.ones_like(th), 1.0)[:3])).cuda()
+    rng = np.random.default_rng(21)
+    I = torch.from_numpy(rng.lognormal(0.0, 1.0, lmn.shape[1])).cuda()
+    c = make_case(nside=4, n_snap=60, n_sat=8, freq=10e6, sigma=50e3, seed=21)
+    uvw, xa, xb = (torch.from_numpy(np.ascontiguousarray(c[k])).cuda() for k in ("uvw", "xa", "xb"))
+    sky_rec = ops.Sky(lmn)
+    monkeypatch.setenv("VISS_QUAD_RECURRENCE", "0")
+    sky_dir = ops.Sky(lmn)
+    monkeypatch.delenv("VISS_QUAD_RECURRENCE")
+    assert sky_rec.pairs()["quad_recurrence"] and sky_dir.pairs()["uses_quads"]
+    assert not sky_dir.pairs()["quad_recurrence"]
+    for mask, track in ((True, False), (True, True), (False, False)):
+        kw = dict(mask=mask, phase_track=track, R_moon=R)
+        Va = sky_rec.forward(I, uvw, xa, xb, **kw).cpu().numpy()
+        Vb = sky_dir.forward(I, uvw, xa, xb, **kw).cpu().numpy()
+        V64 = sky_dir.forward(I, uvw, xa, xb, fp64=True, **kw).cpu().numpy()
+        d, ea, eb = rel_l2(Va, Vb), rel_l2(Va, V64), rel_l2(Vb, V64)
+        print(f"mask={mask} track={track}: recurrent vs direct {d:.2e}; vs FP64 {ea:.2e} / {eb:.2e}", flush=True)
+        assert d < 1e-7 and ea < 1e-6 and eb < 1e-6

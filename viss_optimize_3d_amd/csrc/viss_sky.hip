@@ -12,6 +12,7 @@
 #include <vector>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "viss_internal.h"
@@ -851,7 +852,10 @@ static int sky_build_quads(viss_sky* sky, hipStream_t s) {
       int32_t hr[2] = {0, 0};  // rest pairs, tiles off the phase recurrence
       VISS_HIP(hipMemcpyAsync(hr, counts + 2, sizeof(hr), hipMemcpyDeviceToHost, s));
       VISS_HIP(hipStreamSynchronize(s));
-      sky->qrec = hr[1] == 0;
+      // VISS_QUAD_RECURRENCE=0 keeps the direct phase (A/B runs and the test
+      // that compares the two phases on one sky)
+      const char* rec_env = getenv("VISS_QUAD_RECURRENCE");
+      sky->qrec = hr[1] == 0 && !(rec_env && rec_env[0] == '0');
       const int64_t nrp = hr[0], nrp_pad = (nrp + 63) / 64 * 64, ns_pad = (sky->nsingle + 63) / 64 * 64;
       const int64_t single0 = (sky->npair + 63) / 64 * 64;
       sky->nrtile = (nrp_pad + ns_pad) / 64;
