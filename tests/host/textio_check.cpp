@@ -134,6 +134,83 @@ int main() {
   }
   bad |= check_parse("", "empty");
   bad |= check_parse("   \n\t  ", "whitespace only");
+  // the whole exponent range (random finite bit patterns, subnormals among
+  // them) at 15 to 19 significant digits (the Eisel-Lemire path) and 20 (past it)
+  {
+    std::string t;
+    for (int i = 0; i < 300000; ++i) {
+      uint64_t b = rng();
+      if ((b >> 52 & 0x7FF) == 0x7FF) b &= ~(1ull << 62);  // no inf / nan
+      if (i % 10 == 0) b &= (1ull << 63) | ((1ull << 52) - 1);  // subnormal
+      double x;
+      memcpy(&x, &b, 8);
+      snprintf(buf, sizeof buf, "%.*g ", 15 + i % 6, x);
+      t += buf;
+    }
+    bad |= check_parse(t, "full exponent range, 15-20 digits");
+  }
+  // ties: decimal values exactly halfway between two doubles, at most 19
+  // digits (round half to even), and their neighbours one unit in the last
+  // printed digit away
+  {
+    std::string t;
+    for (int i = 0; i < 40000; ++i) {
+      const int k = i % 5;  // n in [2^(53-k), 2^(54-k)): ulp 2^-k, midpoint n + 2^-(k+1)
+      const uint64_t n = (1ull << (53 - k)) + (rng() >> (11 + k));
+      static const char* frac[] = {".5", ".25", ".125", ".0625", ".03125"};
+      if (k == 0) {  // integers past 2^53: ulp 2, midpoint odd
+        const uint64_t m = (1ull << 53) + 2 * (rng() >> 12) + 1;
+        snprintf(buf, sizeof buf, "%" PRIu64 " %" PRIu64 " %" PRIu64 " ", m, m - 1, m + 1);
+      } else {
+        snprintf(buf, sizeof buf, "%" PRIu64 "%s ", n - 1, frac[k - 1]);
+      }
+      t += buf;
+    }
+    t += "9007199254740993 9007199254740995 4503599627370496.5 4503599627370497.5 "
+         "2.2250738585072011e-308 2.2250738585072012e-308 2.2250738585072014e-308 "
+         "4.9406564584124654e-324 2.4703282292062327e-324 2.4703282292062328e-324 "
+         "1.7976931348623157e308 1.7976931348623158e308 1.7976931348623159e308 "
+         "9999999999999999999e-343 9999999999999999999e288 1e-342 1e-343 1e308 1e309 "
+         "-0 0e400 -0e-400 1844674407370955161e1 18446744073709551615 ";
+    bad |= check_parse(t, "halfway and boundary values");
+  }
+  // parse_numbers_into: the numbers after `skip`, into a vector that is not
+  // zero-filled (multi-threaded and one thread)
+  for (const size_t skip : {(size_t)0, (size_t)3, (size_t)1000000})
+    for (const size_t len : {text.size(), (size_t)4000}) {
+      const std::string sub = text.substr(0, len);
+      const std::vector<double> ref = parse_ref(sub);
+      vissio::HostVec got;
+      vissio::parse_numbers_into(sub.data(), sub.data() + sub.size(), skip, got);
+      bool same = got.size() == (ref.size() > skip ? ref.size() - skip : 0);
+      for (size_t i = 0; same && i < got.size(); ++i) same = same_bits(got[i], ref[i + skip]);
+      if (!same) {
+        printf("FAIL parse_numbers_into skip %zu on %zu bytes\n", skip, len);
+        bad = 1;
+      } else {
+        printf("parse_numbers_into skip %zu on %zu bytes: %zu numbers identical\n", skip, len, got.size());
+      }
+    }
+  // slurp of a file read by several threads (32 MB and more) into a reused buffer
+  {
+    const char* path = "textio_check_big.tmp";
+    std::string want;
+    want.reserve(48u << 20);
+    while (want.size() < (40u << 20)) want += text;
+    FILE* f = fopen(path, "wb");
+    fwrite(want.data(), 1, want.size(), f);
+    fclose(f);
+    std::string got(1000, 'x');
+    const bool ok1 = vissio::slurp(path, got) && got == want;
+    const bool ok2 = vissio::slurp(path, got) && got == want;  // again into the same buffer
+    std::remove(path);
+    if (!ok1 || !ok2) {
+      printf("FAIL slurp of %zu bytes\n", want.size());
+      bad = 1;
+    } else {
+      printf("slurp: %zu bytes identical, twice\n", want.size());
+    }
+  }
   // writer: bytes as snprintf("%.6g") of the float value, or "%.*g" of the double
   for (int prec : {0, 9, 17}) {
     const char* path = "textio_check.tmp";

@@ -22,3 +22,13 @@ def test_host_textio_matches_strtod_and_printf(tmp_path):
     r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+def test_pow5_table_is_generated():
+    """The committed 128-bit power-of-five table is what its generator writes
+    (exact big-integer arithmetic in Python)."""
+    d = os.path.join(ROOT, "viss_optimize_3d_amd", "csrc", "host")
+    out = subprocess.run(["python3", os.path.join(d, "gen_pow5_table.py")], capture_output=True, text=True,
+                         check=True).stdout
+    with open(os.path.join(d, "pow5_table.h")) as f:
+        assert f.read() == out

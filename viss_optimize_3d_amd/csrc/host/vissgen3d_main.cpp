@@ -99,14 +99,15 @@ bool file_exists(const std::string& p) {
 }
 
 // Numbers of a per-day file after `skip` header tokens, grouped by `width`.
-size_t read_rows(const std::string& path, size_t skip, size_t width, std::vector<double>& out) {
-  std::string text;
+// The text buffer is kept per reading thread, so the days after the first
+// read into pages already mapped.
+size_t read_rows(const std::string& path, size_t skip, size_t width, vissio::HostVec& out) {
+  static thread_local std::string text;
   out.clear();
   if (!vissio::slurp(path, text)) return 0;
-  std::vector<double> v = vissio::parse_numbers(text);
-  if (v.size() <= skip) return 0;
-  const size_t rows = (v.size() - skip) / width;
-  out.assign(v.begin() + skip, v.begin() + skip + rows * width);
+  vissio::parse_numbers_into(text.data(), text.data() + text.size(), skip, out);
+  const size_t rows = out.size() / width;
+  out.resize(rows * width);
   return rows;
 }
 
@@ -306,7 +307,7 @@ int main(int argc, char** argv) {
   // runs on the GPU and is written (the text parse is the largest host cost
   // of a day).
   struct DayInput {
-    std::vector<double> uvw_r, xa_r, xb_r, bll_r;
+    vissio::HostVec uvw_r, xa_r, xb_r, bll_r;
     size_t n_uvw = 0, n_xa = 0, n_xb = 0, n_bll = 0;
     double s_read = 0.0;
   };
@@ -342,10 +343,10 @@ int main(int argc, char** argv) {
     DayInput in = pending.get();
     const double s_wait = since(t_wait);
     if (i_day + 1 < todo.size()) pending = std::async(std::launch::async, read_day, todo[i_day + 1]);
-    const std::vector<double>& uvw_r = in.uvw_r;
-    const std::vector<double>& xa_r = in.xa_r;
-    const std::vector<double>& xb_r = in.xb_r;
-    const std::vector<double>& bll_r = in.bll_r;
+    const vissio::HostVec& uvw_r = in.uvw_r;
+    const vissio::HostVec& xa_r = in.xa_r;
+    const vissio::HostVec& xb_r = in.xb_r;
+    const vissio::HostVec& bll_r = in.bll_r;
     const size_t n_uvw = in.n_uvw, n_xa = in.n_xa, n_xb = in.n_xb, n_bll = in.n_bll;
     printf("day %d: uvw_index %zu xyz1_index %zu xyz2_index %zu bll_index %zu\n", day, n_uvw, n_xa, n_xb, n_bll);
     if (n_xa != n_uvw || n_xb != n_uvw || n_bll != n_uvw)
