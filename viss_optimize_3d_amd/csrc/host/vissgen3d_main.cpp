@@ -99,10 +99,9 @@ bool file_exists(const std::string& p) {
 }
 
 // Numbers of a per-day file after `skip` header tokens, grouped by `width`.
-// The text buffer is kept per reading thread, so the days after the first
-// read into pages already mapped.
-size_t read_rows(const std::string& path, size_t skip, size_t width, vissio::HostVec& out) {
-  static thread_local std::string text;
+// `text` is the caller's buffer, kept from file to file and day to day so that
+// the reads after the first go to pages already mapped.
+size_t read_rows(const std::string& path, size_t skip, size_t width, std::string& text, vissio::HostVec& out) {
   out.clear();
   if (!vissio::slurp(path, text)) return 0;
   vissio::parse_numbers_into(text.data(), text.data() + text.size(), skip, out);
@@ -313,14 +312,17 @@ int main(int argc, char** argv) {
   };
   using clk = std::chrono::steady_clock;
   const auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
-  const auto read_day = [&o, &since](int day) {
+  // one read runs at a time (the next day's starts after this day's input is
+  // taken), so the days share one text buffer
+  std::string day_text;
+  const auto read_day = [&o, &since, &day_text](int day) {
     const clk::time_point t0 = clk::now();
     const std::string tag = std::to_string(day) + "day1M.txt";
     DayInput in;
-    in.n_uvw = read_rows(o.in_dir + "uvw" + tag, 3, 3, in.uvw_r);
-    in.n_xa = read_rows(o.in_dir + "xyza" + tag, 3, 3, in.xa_r);
-    in.n_xb = read_rows(o.in_dir + "xyzb" + tag, 3, 3, in.xb_r);
-    in.n_bll = read_rows(o.in_dir + "bll" + tag, 1, 1, in.bll_r);
+    in.n_uvw = read_rows(o.in_dir + "uvw" + tag, 3, 3, day_text, in.uvw_r);
+    in.n_xa = read_rows(o.in_dir + "xyza" + tag, 3, 3, day_text, in.xa_r);
+    in.n_xb = read_rows(o.in_dir + "xyzb" + tag, 3, 3, day_text, in.xb_r);
+    in.n_bll = read_rows(o.in_dir + "bll" + tag, 1, 1, day_text, in.bll_r);
     in.s_read = since(t0);
     return in;
   };
