@@ -360,20 +360,25 @@ int main(int argc, char** argv) {
     const size_t amount = rows / 2;  // vissgen3d_noblockage.cu:400 (integer division)
     std::fill(hC.begin(), hC.end(), 0.0);
     if (rows > 0) {
-      // SoA host arrays: all rows, and the first `amount` rows for the forward
-      std::vector<double> uvw(3 * rows), xa(3 * rows, 0.0), xb(3 * rows, 0.0), w(rows);
-      std::vector<float> bll(rows, 0.0f);
-      for (size_t k = 0; k < rows; ++k) {
+      // SoA host arrays: all rows, and the first `amount` rows for the forward.
+      // The loops over rows run on the host's threads, and every element is
+      // written once (missing rows as 0), so no array is zero-filled first.
+      const int64_t nrow = (int64_t)rows, namt = (int64_t)amount;
+      vissio::HostVec uvw(3 * rows), xa(3 * rows), xb(3 * rows), w(rows);
+      std::vector<float, vissio::NoInitAlloc<float>> bll(rows);
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < nrow; ++k) {
         for (int c = 0; c < 3; ++c) {
           uvw[c * rows + k] = uvw_r[3 * k + c];
-          if (k < n_xa) xa[c * rows + k] = xa_r[3 * k + c];
-          if (k < n_xb) xb[c * rows + k] = xb_r[3 * k + c];
+          xa[c * rows + k] = (size_t)k < n_xa ? xa_r[3 * k + c] : 0.0;
+          xb[c * rows + k] = (size_t)k < n_xb ? xb_r[3 * k + c] : 0.0;
         }
         w[k] = uvw_r[3 * k + 2];
-        if (k < n_bll) bll[k] = (float)bll_r[k];
+        bll[k] = (size_t)k < n_bll ? (float)bll_r[k] : 0.0f;
       }
-      std::vector<double> uvwf(3 * amount), xaf(3 * amount), xbf(3 * amount);
-      for (size_t k = 0; k < amount; ++k)
+      vissio::HostVec uvwf(3 * amount), xaf(3 * amount), xbf(3 * amount);
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < namt; ++k)
         for (int c = 0; c < 3; ++c) {
           uvwf[c * amount + k] = uvw[c * rows + k];
           xaf[c * amount + k] = xa[c * rows + k];
@@ -382,28 +387,36 @@ int main(int argc, char** argv) {
       // shell count: the reference's nr, extended to the largest shell index present
       int nr = nr_ref;
       int64_t n_over = 0;
-      for (size_t k = 0; k < rows; ++k) {
+      const bool extend = o.extend_nr;
+#pragma omp parallel for schedule(static) reduction(+ : n_over) reduction(max : nr)
+      for (int64_t k = 0; k < nrow; ++k) {
         const int g = shell_index(bll[k]);
         if (g > nr_ref) ++n_over;
-        if (o.extend_nr && g > nr) nr = g;
+        if (extend && g > nr) nr = g;
       }
       if (n_over)
         printf("day %d: %lld rows beyond shell %d (%s)\n", day, (long long)n_over, nr_ref,
                o.extend_nr ? "nr extended" : "weight 0");
       // Hermitian fold check: row k + amount = (-b_k, same bll, satellites swapped or equal)
       bool fold = rows == 2 * amount && amount > 0;
-      for (size_t k = 0; fold && k < amount; ++k) {
-        const size_t j = k + amount;
-        for (int c = 0; c < 3; ++c) fold = fold && uvw[c * rows + j] == -uvw[c * rows + k];
-        fold = fold && bll[j] == bll[k];
-        if (image_mask && fold) {
-          bool same = true, swapped = true;
-          for (int c = 0; c < 3; ++c) {
-            same = same && xa[c * rows + j] == xa[c * rows + k] && xb[c * rows + j] == xb[c * rows + k];
-            swapped = swapped && xa[c * rows + j] == xb[c * rows + k] && xb[c * rows + j] == xa[c * rows + k];
+      if (fold) {
+        int64_t n_bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : n_bad)
+        for (int64_t k = 0; k < namt; ++k) {
+          const size_t j = (size_t)k + amount;
+          bool ok = bll[j] == bll[k];
+          for (int c = 0; c < 3; ++c) ok = ok && uvw[c * rows + j] == -uvw[c * rows + k];
+          if (image_mask && ok) {
+            bool same = true, swapped = true;
+            for (int c = 0; c < 3; ++c) {
+              same = same && xa[c * rows + j] == xa[c * rows + k] && xb[c * rows + j] == xb[c * rows + k];
+              swapped = swapped && xa[c * rows + j] == xb[c * rows + k] && xb[c * rows + j] == xa[c * rows + k];
+            }
+            ok = same || swapped;
           }
-          fold = same || swapped;
+          n_bad += ok ? 0 : 1;
         }
+        fold = n_bad == 0;
       }
       s_prep = since(t_gpu);
       const clk::time_point t_h2d = clk::now();
